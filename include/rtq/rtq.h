@@ -1,0 +1,211 @@
+/*
+ * rtq — C ABI of the MI355X (gfx950) ray-query engine.
+ *
+ * The reference has no FFI: its seam is the srcCommon C++ scene API
+ * (cr::BVH built in its constructor, bvh.hpp:84-124 / bvh.cpp:11-24), the
+ * pre-order flatten done by the OpenGL scene (scene.cpp:189-208) and the GLSL
+ * traversal dispatched per frame (application.cpp:220-246, raytracer.glsl:246-331).
+ * Every entry point below replaces one of those, cited on its declaration.
+ *
+ * Conventions
+ *   - Every call returns an rtq_status; nothing exits the process (the
+ *     reference calls exit() through ErrorHandler::handle, errorHandler.cpp:13-45).
+ *     rtq_last_error() gives the message of the last failing call on a context.
+ *   - "_dev" pointers are device (HBM) pointers on the context's device;
+ *     everything else is host memory.  Streams are hipStream_t passed as void*;
+ *     NULL means the context's own stream.
+ *   - A context is bound to one device and is not thread-safe.
+ *   - Plain pointers and sizes only: no C++ or torch types cross this boundary.
+ */
+#ifndef RTQ_RTQ_H
+#define RTQ_RTQ_H
+
+#include "scene_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The library is built with hidden visibility; only these declarations export. */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
+#define RTQ_ABI_VERSION 1
+
+typedef struct rtq_ctx rtq_ctx;
+
+typedef enum rtq_status {
+    RTQ_OK = 0,
+    RTQ_EINVAL = 1,      /* bad argument (null pointer, N = 0, non-finite vertex, ...) */
+    RTQ_ENOMEM = 2,      /* device allocation failed */
+    RTQ_EHIP = 3,        /* a HIP runtime call failed */
+    RTQ_ECAPACITY = 4,   /* caller buffer too small / size beyond a documented limit */
+    RTQ_EINTERNAL = 5,   /* an internal invariant failed (e.g. a PLOC pass made no merge) */
+    RTQ_ENOTBUILT = 6    /* scene has no BVH yet */
+} rtq_status;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces the GL context + SSBO set created by glr::Scene (scene.cpp:69-108). */
+rtq_status rtq_create(rtq_ctx** out, int hip_device);
+void rtq_destroy(rtq_ctx* ctx);
+const char* rtq_last_error(const rtq_ctx* ctx);
+const char* rtq_status_string(rtq_status s);
+int rtq_abi_version(void);
+/* Synchronise the context's stream (all work issued with stream == NULL). */
+rtq_status rtq_synchronize(rtq_ctx* ctx);
+
+/* ---- scene -------------------------------------------------------------- */
+
+/* Upload a scene given in the reference layouts (triangles: 64-B TriangleGPU,
+ * models: 68-B MeshModelGPU).  Replaces Scene::bindSSBO's triangle/model
+ * uploads (scene.cpp:130-156) and the input copy of BVH::BVH (bvh.cpp:11-17).
+ * Vertices are transformed by their model matrix once, here, in the order
+ * glm's mat4 * vec4 uses (bvh.cpp:238-240), instead of per intersection
+ * (raytracer.glsl:107-109).  n_tris has no 65,536 cap (triangle.hpp:21).
+ * Rejects n_tris == 0, model ids >= n_models and non-finite vertices.
+ * Host arrays; the library copies them. */
+rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_tris,
+                            const rtq_mesh_model* models, uint32_t n_models,
+                            uint32_t* scene_id);
+rtq_status rtq_scene_release(rtq_ctx* ctx, uint32_t scene_id);
+
+/* ---- BVH build ---------------------------------------------------------- */
+
+/* Build flags. */
+#define RTQ_BUILD_REF_PADDED_BOUNDS 0x1u  /* scene bounds also cover the origin when
+                                             n_tris < 65,536, as the reference does by
+                                             scanning its zero-padded triangle vector
+                                             (bvh.cpp:237, scene.cpp:26-40) */
+
+typedef struct rtq_build_params {
+    uint32_t search_radius;   /* PLOC neighbour window radius (bvh.hpp:66: 16); 1..32 */
+    uint32_t max_leaf_size;   /* triangles per traversal leaf, 1..8 (the reference
+                                 node array keeps one triangle per leaf regardless) */
+    uint32_t flags;           /* RTQ_BUILD_* */
+    uint32_t reserved;
+} rtq_build_params;
+
+void rtq_build_params_default(rtq_build_params* p);
+
+/* GPU build: scene bounds (bvh.cpp:235-251), circumscribed cube (253-303),
+ * centroids and Morton codes (305-372), stable (code, index) sort (214-232),
+ * PLOC (26-210) and the pre-order flatten (scene.cpp:189-208), then the
+ * traversal layout.  Replaces BVH::BVH (bvh.cpp:11-24). */
+rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params* params);
+
+typedef struct rtq_bvh_info {
+    uint32_t n_tris;
+    uint32_t n_ref_nodes;        /* 2N - 1 */
+    uint32_t n_trav_nodes;       /* traversal (two-child) nodes */
+    uint32_t depth;              /* max leaf depth of the binary tree (root = 0) */
+    uint32_t ploc_iterations;
+    uint32_t max_leaf_size;
+    float box_pad;               /* outward padding of traversal boxes (world units) */
+    float build_ms;              /* device time of the last build */
+} rtq_bvh_info;
+
+rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out);
+
+/* Pre-order BVH_NodeGPU array, root at 0, left child at i+1, exactly what
+ * Scene::getBVH_NodesToGPUData returns (scene.cpp:189-208).  Host buffer of
+ * `capacity` nodes; *n_nodes receives 2N-1. */
+rtq_status rtq_bvh_export_ref(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_node* nodes,
+                              uint64_t capacity, uint64_t* n_nodes);
+
+/* The build's cluster arrays in the form of BVH_Params (bvh.hpp:44-63):
+ * cluster c < N is the leaf for sorted position c, c >= N are merges in the
+ * order they were made (ids assigned in increasing position order within a
+ * PLOC pass, where the reference numbers them in OpenMP critical-section order,
+ * bvh.cpp:175-179).  Missing values (leaf children, root parent) are
+ * 0xFFFFFFFF.  Every pointer may be NULL; each non-NULL array holds
+ * `capacity` >= 2N-1 entries, except sorted_indices / sorted_codes (N). */
+rtq_status rtq_bvh_export_clusters(rtq_ctx* ctx, uint32_t scene_id, uint64_t capacity,
+                                   rtq_aabb* boxes, uint32_t* triangle_id,
+                                   uint32_t* left, uint32_t* right, uint32_t* parent,
+                                   uint32_t* sorted_indices, uint32_t* sorted_codes);
+
+/* Unsorted 30-bit Morton codes of the uploaded triangles (bvh.cpp:330-348),
+ * host output of n_tris entries. */
+rtq_status rtq_morton_codes(rtq_ctx* ctx, uint32_t scene_id, uint32_t flags, uint32_t* codes);
+
+/* Stable LSD radix sort of (key, value) pairs by key on device memory, in
+ * place; with iota values it equals std::sort on (code, index) pairs
+ * (bvh.cpp:223-231).  key_bits <= 32 selects how many low key bits are sorted. */
+rtq_status rtq_sort_pairs(rtq_ctx* ctx, uint32_t* keys_dev, uint32_t* vals_dev, uint64_t n,
+                          uint32_t key_bits, void* stream);
+
+/* Exclusive prefix sum of u32 (wrapping) on device memory; the scan behind
+ * plocPrefixScan (bvh.cpp:125-148) and the reference's digit-count scan
+ * (prefixSumOfGlobalDigitCounts.glsl:24-69) when applied per group. */
+rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t* out_dev,
+                                  uint64_t n, void* stream);
+
+/* ---- trace -------------------------------------------------------------- */
+
+#define RTQ_TRACE_CLOSEST     0x0u  /* closest hit, ties -> lowest triangle id */
+#define RTQ_TRACE_ANY         0x1u  /* any hit (occlusion), ends at the first hit found */
+#define RTQ_TRACE_LEAF_REFMT  0x2u  /* leaf test = the reference's Moller-Trumbore
+                                       (raytracer.glsl:102-147: back-face culled,
+                                       |det| < 1e-4 rejected, vertices 1/2 swapped);
+                                       default leaf test is watertight FP32 */
+
+/* Trace n_rays rays against the scene's BVH.  Replaces the traversal of
+ * getClosestHitBVH (raytracer.glsl:246-295) dispatched by drawOneFrame
+ * (application.cpp:220-246).  Device buffers, asynchronous on `stream`. */
+rtq_status rtq_trace(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays,
+                     uint32_t flags, rtq_hit* hits_dev, void* stream);
+
+/* Same trace, also writing per-ray traversal counters (node fetches, triangle
+ * tests) for the algorithmic-byte count.  Counter arrays may be NULL. */
+rtq_status rtq_trace_stats(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev,
+                           uint64_t n_rays, uint32_t flags, rtq_hit* hits_dev,
+                           uint32_t* node_visits_dev, uint32_t* tri_tests_dev, void* stream);
+
+/* Synchronous convenience form on host arrays (copies in, traces, copies out). */
+rtq_status rtq_trace_host(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays, uint64_t n_rays,
+                          uint32_t flags, rtq_hit* hits);
+
+/* ---- ray generation (caller side of the path) --------------------------- */
+
+/* Pinhole camera given by precomputed basis vectors: for pixel (i, j) of a
+ * width x height image, ray index j*width + i,
+ *   s = i/width - 0.5, t = j/height - 0.5   (no half-pixel offset, as
+ *                                            raytracer.glsl:303-307)
+ *   d = normalize(right*s + up*t + forward), o = eye, interval [0, +inf).
+ * right/up are scaled to the image plane at unit distance along forward
+ * (the reference's plane width/height over near, camera.cpp:52-66). */
+typedef struct rtq_pinhole {
+    float eye[3];
+    float right[3];
+    float up[3];
+    float forward[3];
+    uint32_t width;
+    uint32_t height;
+} rtq_pinhole;
+
+rtq_status rtq_raygen_pinhole(rtq_ctx* ctx, const rtq_pinhole* cam, rtq_ray* rays_dev, void* stream);
+
+/* Secondary rays from primary hits (hit point p = o + t*d, geometric normal
+ * facing the incoming ray, origin p + eps*n).  A primary miss yields an empty
+ * ray (tmin = 1, tmax = 0) that traces as a miss.
+ *  - diffuse: one cosine-weighted direction per hit, seeded counter-based RNG;
+ *  - shadow:  n_lights rays per hit towards point lights (host array xyz),
+ *             d = light - origin, interval [0, 1]; ray index = light*n + i. */
+rtq_status rtq_raygen_diffuse(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* prim_dev,
+                              const rtq_hit* hits_dev, uint64_t n, uint64_t seed, float eps,
+                              rtq_ray* out_dev, void* stream);
+rtq_status rtq_raygen_shadow(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* prim_dev,
+                             const rtq_hit* hits_dev, uint64_t n, const float* lights,
+                             uint32_t n_lights, float eps, rtq_ray* out_dev, void* stream);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* RTQ_RTQ_H */

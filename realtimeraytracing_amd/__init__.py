@@ -1,0 +1,12 @@
+"""rtq: an MI355X (gfx950) ray-query engine for PLOC-built BVHs.
+
+The product is the C ABI in include/rtq/rtq.h, implemented by the HIP library
+realtimeraytracing_amd/librtq.so.  This package holds its sources (csrc/), the
+build script, ctypes bindings used by tests and the benchmark, and the
+synthetic scene generators.
+"""
+from .rtq import (  # noqa: F401
+    AABB_DTYPE, EXPORTED, HIT_DTYPE, MODEL_DTYPE, NODE_DTYPE, RAY_DTYPE, TRIANGLE_DTYPE,
+    RTQ_BUILD_REF_PADDED_BOUNDS, RTQ_TRACE_ANY, RTQ_TRACE_CLOSEST, RTQ_TRACE_LEAF_REFMT,
+    BuildParams, BvhInfo, Context, Pinhole, RtqError, Scene, load_library,
+)

@@ -1,0 +1,650 @@
+// GPU BVH build: scene bounds -> circumscribed cube -> Morton codes -> stable
+// radix sort -> PLOC passes -> pre-order flatten -> traversal layout.
+//
+// Float arithmetic follows the reference's evaluation order exactly (built with
+// -ffp-contract=off, std::min/std::max written as the same compares), so codes,
+// cluster boxes and the flattened node array are bit-identical to a serial run
+// of bvh.cpp / scene.cpp on the same input.
+#include <cmath>
+#include <cstring>
+
+#include "rtq_internal.hpp"
+
+namespace rtq {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ float canon0(float x) { return x == 0.0f ? 0.0f : x; }
+
+// glm mat4 * vec4 as evaluated at bvh.cpp:238-240 / triangle.cpp:33:
+// (m0*x + m1*y) + (m2*z + m3*w), column-major m.
+__device__ __forceinline__ void mat_vec(const float* m, float x, float y, float z, float w,
+                                        float* out) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float a0 = m[0 + r] * x + m[4 + r] * y;
+        float a1 = m[8 + r] * z + m[12 + r] * w;
+        out[r] = a0 + a1;
+    }
+}
+
+__device__ __forceinline__ void world_vertex(const DevModel& md, const rtq_vec4& p, float* out) {
+    float m[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = md.m[i];
+    mat_vec(m, p.x, p.y, p.z, p.w, out);
+}
+
+// Triangle box, bvh.cpp:383-399.
+__device__ __forceinline__ void tri_box(const rtq_triangle& t, const DevModel& md, float* lo, float* hi) {
+    float p0[3], p1[3], p2[3];
+    world_vertex(md, t.p0, p0);
+    world_vertex(md, t.p1, p1);
+    world_vertex(md, t.p2, p2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = smin(p0[k], smin(p1[k], p2[k]));
+        hi[k] = smax(p0[k], smax(p1[k], p2[k]));
+    }
+}
+
+// ------------------------------------------------------------- scene bounds
+
+__global__ __launch_bounds__(kThreads) void k_bounds_partial(const rtq_triangle* __restrict__ tris,
+                                                             const DevModel* __restrict__ models,
+                                                             uint32_t n, float* __restrict__ partial) {
+    __shared__ float s[6][kThreads];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        float a[3], b[3];
+        tri_box(tris[i], models[tris[i].model_id], a, b);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { lo[k] = smin(lo[k], a[k]); hi[k] = smax(hi[k], b[k]); }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = lo[k]; s[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                s[k][threadIdx.x] = smin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = smax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Reduces the partial boxes, then (one thread) the circumscribed cube of
+// bvh.cpp:253-303.  out[0] = scene box, out[1] = cube.
+__global__ __launch_bounds__(kThreads) void k_bounds_final(const float* __restrict__ partial, uint32_t nb,
+                                                           uint32_t n, uint32_t pad_origin,
+                                                           rtq_aabb* __restrict__ out) {
+    __shared__ float s[6][kThreads];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t b = threadIdx.x; b < nb; b += kThreads) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = smin(lo[k], partial[b * 6 + k]);
+            hi[k] = smax(hi[k], partial[b * 6 + 3 + k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = lo[k]; s[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                s[k][threadIdx.x] = smin(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = smax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    rtq_aabb b{};
+    for (int k = 0; k < 3; ++k) { b.min[k] = s[k][0]; b.max[k] = s[3 + k][0]; }
+    if (pad_origin && n < 65536u) {
+        for (int k = 0; k < 3; ++k) { b.max[k] = smax(b.max[k], 0.0f); b.min[k] = smin(b.min[k], 0.0f); }
+    }
+    for (int k = 0; k < 3; ++k) { b.min[k] = canon0(b.min[k]); b.max[k] = canon0(b.max[k]); }
+    out[0] = b;
+    rtq_aabb c = b;
+    float dist_x = b.max[0] - b.min[0];
+    float dist_y = b.max[1] - b.min[1];
+    float dist_z = b.max[2] - b.min[2];
+    float max_dist = 0.0f;
+    int axis;
+    if (dist_x > max_dist) { max_dist = dist_x; axis = 0; }
+    else if (dist_y > max_dist) { max_dist = dist_y; axis = 1; }
+    else { max_dist = dist_z; axis = 2; }
+    float delta;
+    if (axis == 0) {
+        delta = (max_dist - dist_y) / 2.0f; c.max[1] += delta; c.min[1] -= delta;
+        delta = (max_dist - dist_z) / 2.0f; c.max[2] += delta; c.min[2] -= delta;
+    } else if (axis == 1) {
+        delta = (max_dist - dist_x) / 2.0f; c.max[0] += delta; c.min[0] -= delta;
+        delta = (max_dist - dist_z) / 2.0f; c.max[2] += delta; c.min[2] -= delta;
+    } else {
+        delta = (max_dist - dist_x) / 2.0f; c.max[0] += delta; c.min[0] -= delta;
+        delta = (max_dist - dist_y) / 2.0f; c.max[1] += delta; c.min[1] -= delta;
+    }
+    out[1] = c;
+}
+
+// ------------------------------------------------------------- Morton codes
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quantize10(float x) {
+    x = smin(smax(x * 1024.0f, 0.0f), 1023.0f);
+    if (x != x) return 0u;   // zero-extent axis: the reference cast is undefined
+    return (uint32_t)x;
+}
+
+// bvh.cpp:305-372 for one triangle.
+__global__ __launch_bounds__(kThreads) void k_morton(const rtq_triangle* __restrict__ tris,
+                                                     const DevModel* __restrict__ models, uint32_t n,
+                                                     const rtq_aabb* __restrict__ bounds,
+                                                     uint32_t* __restrict__ codes,
+                                                     uint32_t* __restrict__ idx) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const rtq_aabb cube = bounds[1];
+    const rtq_triangle t = tris[i];
+    const DevModel& md = models[t.model_id];
+    const float third = 1.0f / 3.0f;
+    float ms[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) ms[k] = third * md.m[k];
+    float sx = (t.p0.x + t.p1.x) + t.p2.x;
+    float sy = (t.p0.y + t.p1.y) + t.p2.y;
+    float sz = (t.p0.z + t.p1.z) + t.p2.z;
+    float sw = (t.p0.w + t.p1.w) + t.p2.w;
+    float c[3];
+    mat_vec(ms, sx, sy, sz, sw, c);
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float len = cube.max[k] - cube.min[k];
+        q[k] = quantize10((c[k] - cube.min[k]) / len);
+    }
+    codes[i] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    if (idx) idx[i] = i;
+}
+
+// --------------------------------------------------------------------- PLOC
+
+struct Box6 {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ float merged_sa(const Box6& a, const Box6& b) {
+    // AABB::merge then AABB::getSurfaceArea (bvh.cpp:378-381, 401-413)
+    float dx = smax(a.hi[0], b.hi[0]) - smin(a.lo[0], b.lo[0]);
+    float dy = smax(a.hi[1], b.hi[1]) - smin(a.lo[1], b.lo[1]);
+    float dz = smax(a.hi[2], b.hi[2]) - smin(a.lo[2], b.lo[2]);
+    return 2.0f * ((dx * dy + dy * dz) + dz * dx);
+}
+
+__global__ __launch_bounds__(kThreads) void k_cluster_init(uint32_t n, uint32_t* __restrict__ cparent,
+                                                           uint32_t* __restrict__ cleft,
+                                                           uint32_t* __restrict__ cright,
+                                                           uint32_t* __restrict__ ctri) {
+    const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= 2 * n - 1) return;
+    cparent[c] = kNone;
+    cleft[c] = kNone;
+    cright[c] = kNone;
+    ctri[c] = 0;
+}
+
+// plocPreprocessing, bvh.cpp:26-46.
+__global__ __launch_bounds__(kThreads) void k_leaf_init(const rtq_triangle* __restrict__ tris,
+                                                        const DevModel* __restrict__ models,
+                                                        const uint32_t* __restrict__ sorted_idx,
+                                                        uint32_t n, rtq_aabb* __restrict__ cbox,
+                                                        uint32_t* __restrict__ ctri,
+                                                        uint32_t* __restrict__ ccount,
+                                                        uint32_t* __restrict__ c_in,
+                                                        Box6* __restrict__ b_in) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t ti = sorted_idx[i];
+    const rtq_triangle t = tris[ti];
+    Box6 b;
+    tri_box(t, models[t.model_id], b.lo, b.hi);
+    rtq_aabb a{};
+    for (int k = 0; k < 3; ++k) { a.min[k] = b.lo[k]; a.max[k] = b.hi[k]; }
+    cbox[i] = a;
+    ctri[i] = ti;
+    ccount[i] = 1;
+    c_in[i] = i;
+    b_in[i] = b;
+}
+
+// plocNearestNeighborSearch, bvh.cpp:193-210: window [i-r, i+r] of the active
+// cluster list staged through LDS; strict '<' keeps the lowest j on ties.
+__global__ __launch_bounds__(kThreads) void k_ploc_nn(const Box6* __restrict__ b_in, uint32_t iter,
+                                                      uint32_t radius, uint32_t* __restrict__ nn) {
+    __shared__ Box6 s_box[kThreads + 64];
+    const uint32_t base = blockIdx.x * kThreads;
+    const uint32_t lo = base > radius ? base - radius : 0u;
+    const uint32_t hi = min(iter, base + kThreads + radius);
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += kThreads) s_box[j - lo] = b_in[j];
+    __syncthreads();
+    const uint32_t i = base + threadIdx.x;
+    if (i >= iter) return;
+    const Box6 bi = s_box[i - lo];
+    const uint32_t start = i > radius ? i - radius : 0u;
+    const uint32_t end = min(i + radius + 1, iter);
+    float best = INFINITY;
+    uint32_t arg = kNone;
+    for (uint32_t j = start; j < end; ++j) {
+        if (j == i) continue;
+        float d = merged_sa(bi, s_box[j - lo]);
+        if (d < best) { best = d; arg = j; }
+    }
+    if (arg != kNone) nn[i] = arg;
+}
+
+// Per position: x = 1 if this position starts a merge (mutual pair, lower
+// index), y = 1 if a cluster stays at this position after merging.
+__global__ __launch_bounds__(kThreads) void k_ploc_flags(const uint32_t* __restrict__ nn, uint32_t iter,
+                                                         uint2* __restrict__ flags) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= iter) return;
+    const uint32_t j = nn[i];
+    const bool mutual = (j < iter) && (j != i) && nn[j] == i;
+    flags[i] = make_uint2((mutual && i < j) ? 1u : 0u, (mutual && j < i) ? 0u : 1u);
+}
+
+// plocMerging + plocCompaction (bvh.cpp:150-191): merged cluster ids are
+// n_total + (number of merges at lower positions), deterministic.
+__global__ __launch_bounds__(kThreads) void k_ploc_merge(
+    uint32_t iter, uint32_t n_total, const uint32_t* __restrict__ nn, const uint2* __restrict__ flags,
+    const uint2* __restrict__ offs, const uint32_t* __restrict__ c_in, const Box6* __restrict__ b_in,
+    uint32_t* __restrict__ c_out, Box6* __restrict__ b_out, rtq_aabb* __restrict__ cbox,
+    uint32_t* __restrict__ cleft, uint32_t* __restrict__ cright, uint32_t* __restrict__ cparent,
+    uint32_t* __restrict__ ccount, uint32_t* __restrict__ ctri) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= iter) return;
+    const uint2 f = flags[i];
+    if (!f.y) return;
+    const uint2 o = offs[i];
+    uint32_t cid = c_in[i];
+    Box6 b = b_in[i];
+    if (f.x) {
+        const uint32_t j = nn[i];
+        const uint32_t cj = c_in[j];
+        const Box6 bj = b_in[j];
+        const uint32_t id = n_total + o.x;
+        Box6 m;
+        rtq_aabb a{};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            m.lo[k] = smin(b.lo[k], bj.lo[k]);
+            m.hi[k] = smax(b.hi[k], bj.hi[k]);
+            a.min[k] = m.lo[k];
+            a.max[k] = m.hi[k];
+        }
+        cbox[id] = a;
+        cleft[id] = cid;
+        cright[id] = cj;
+        cparent[cid] = id;
+        cparent[cj] = id;
+        ccount[id] = ccount[cid] + ccount[cj];
+        ctri[id] = 0;
+        cid = id;
+        b = m;
+    }
+    c_out[o.y] = cid;
+    b_out[o.y] = b;
+}
+
+// --------------------------------------------------------------- flattening
+
+__global__ void k_root_init(uint32_t root, uint32_t* pos, uint32_t* fl, uint32_t* dep) {
+    pos[root] = 0;
+    fl[root] = 0;
+    dep[root] = 0;
+}
+
+// Top-down pre-order placement for the clusters created in one PLOC pass
+// (their parents were created in later passes, so are already placed):
+// left child right after the parent, right child after the left subtree
+// (scene.cpp:189-201); fl = rank of the subtree's first leaf.
+__global__ __launch_bounds__(kThreads) void k_place_pass(uint32_t first, uint32_t count,
+                                                         const uint32_t* __restrict__ cleft,
+                                                         const uint32_t* __restrict__ cright,
+                                                         const uint32_t* __restrict__ ccount,
+                                                         uint32_t* __restrict__ pos,
+                                                         uint32_t* __restrict__ fl,
+                                                         uint32_t* __restrict__ dep) {
+    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t p = first + k;
+    const uint32_t l = cleft[p], r = cright[p];
+    const uint32_t pp = pos[p], pf = fl[p], pd = dep[p];
+    const uint32_t lc = ccount[l];
+    pos[l] = pp + 1;
+    pos[r] = pp + 2 * lc;
+    fl[l] = pf;
+    fl[r] = pf + lc;
+    dep[l] = pd + 1;
+    dep[r] = pd + 1;
+}
+
+// Reference node array (scene.cpp:189-208) + leaf-ordered traversal triangles.
+__global__ __launch_bounds__(kThreads) void k_write_ref(
+    uint32_t n_clusters, const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ ctri,
+    const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
+    const uint32_t* __restrict__ pos, const uint32_t* __restrict__ fl, const uint32_t* __restrict__ dep,
+    const float4* __restrict__ tri_world, rtq_bvh_node* __restrict__ ref, float4* __restrict__ tris,
+    uint32_t* __restrict__ max_depth) {
+    const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= n_clusters) return;
+    rtq_bvh_node nd{};
+    nd.box = cbox[c];
+    nd.box.pad0_ = 0.0f;
+    nd.box.pad1_ = 0.0f;
+    const bool leaf = cleft[c] == kNone;
+    nd.triangle_id = leaf ? ctri[c] : 0u;
+    nd.left_child = leaf ? 0u : pos[cleft[c]];
+    nd.right_child = leaf ? 0u : pos[cright[c]];
+    nd.pad_ = 0u;
+    ref[pos[c]] = nd;
+    if (leaf) {
+        const uint32_t t = ctri[c];
+        const uint64_t dst = 3ull * fl[c];
+        tris[dst + 0] = tri_world[3ull * t + 0];
+        tris[dst + 1] = tri_world[3ull * t + 1];
+        tris[dst + 2] = tri_world[3ull * t + 2];
+        atomicMax(max_depth, dep[c]);
+    }
+}
+
+__device__ __forceinline__ bool is_trav_node(uint32_t c, uint32_t root, uint32_t max_leaf,
+                                             const uint32_t* cleft, const uint32_t* ccount) {
+    return cleft[c] != kNone && (ccount[c] > max_leaf || c == root);
+}
+
+__global__ __launch_bounds__(kThreads) void k_mark_trav(uint32_t n_clusters, uint32_t root, uint32_t max_leaf,
+                                                        const uint32_t* __restrict__ cleft,
+                                                        const uint32_t* __restrict__ ccount,
+                                                        const uint32_t* __restrict__ pos,
+                                                        uint32_t* __restrict__ flag) {
+    const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= n_clusters) return;
+    flag[pos[c]] = is_trav_node(c, root, max_leaf, cleft, ccount) ? 1u : 0u;
+}
+
+// Traversal node (64 B): child boxes widened outward by `pad`, interleaved as
+//   [c0.lo.x c0.hi.x c0.lo.y c0.hi.y] [c0.lo.z c0.hi.z c1.lo.x c1.hi.x]
+//   [c1.lo.y c1.hi.y c1.lo.z c1.hi.z] [ref0 ref1 0 0]
+__device__ __forceinline__ void child_desc(uint32_t ch, uint32_t root, uint32_t max_leaf,
+                                           const rtq_aabb* cbox, const uint32_t* cleft,
+                                           const uint32_t* ccount, const uint32_t* pos,
+                                           const uint32_t* fl, const uint32_t* bidx, float pad,
+                                           float* lo, float* hi, uint32_t* ref) {
+    const rtq_aabb a = cbox[ch];
+    for (int k = 0; k < 3; ++k) { lo[k] = a.min[k] - pad; hi[k] = a.max[k] + pad; }
+    if (is_trav_node(ch, root, max_leaf, cleft, ccount)) *ref = bidx[pos[ch]];
+    else *ref = leaf_ref(fl[ch], ccount[ch]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_write_trav(
+    uint32_t n_clusters, uint32_t root, uint32_t max_leaf, float pad, const rtq_aabb* __restrict__ cbox,
+    const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
+    const uint32_t* __restrict__ ccount, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ fl,
+    const uint32_t* __restrict__ bidx, float4* __restrict__ nodes) {
+    const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= n_clusters) return;
+    if (!is_trav_node(c, root, max_leaf, cleft, ccount)) return;
+    float l0[3], h0[3], l1[3], h1[3];
+    uint32_t r0, r1;
+    child_desc(cleft[c], root, max_leaf, cbox, cleft, ccount, pos, fl, bidx, pad, l0, h0, &r0);
+    child_desc(cright[c], root, max_leaf, cbox, cleft, ccount, pos, fl, bidx, pad, l1, h1, &r1);
+    float4* o = nodes + 4ull * bidx[pos[c]];
+    o[0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
+    o[1] = make_float4(l0[2], h0[2], l1[0], h1[0]);
+    o[2] = make_float4(l1[1], h1[1], l1[2], h1[2]);
+    o[3] = make_float4(__uint_as_float(r0), __uint_as_float(r1), 0.0f, 0.0f);
+}
+
+// Single-triangle scene: one node whose two children are the same leaf.
+__global__ void k_write_single(const rtq_aabb* __restrict__ cbox, float pad, float4* __restrict__ nodes) {
+    const rtq_aabb a = cbox[0];
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) { lo[k] = a.min[k] - pad; hi[k] = a.max[k] + pad; }
+    const uint32_t r = leaf_ref(0, 1);
+    nodes[0] = make_float4(lo[0], hi[0], lo[1], hi[1]);
+    nodes[1] = make_float4(lo[2], hi[2], lo[0], hi[0]);
+    nodes[2] = make_float4(lo[1], hi[1], lo[2], hi[2]);
+    nodes[3] = make_float4(__uint_as_float(r), __uint_as_float(r), 0.0f, 0.0f);
+}
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
+
+// Shared front half of the build and of rtq_morton_codes: scene box + cube.
+rtq_status compute_bounds(rtq_ctx* ctx, Scene* s, uint32_t pad_origin, rtq_aabb* d_bounds,
+                          hipStream_t st) {
+    const uint32_t nb = std::min<uint32_t>(blocks_for(s->n_tris), 1024u);
+    float* partial = nullptr;
+    RTQ_HIP(ctx, dev_alloc(&partial, 6ull * nb));
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kThreads), 0, st, s->tris_ref, s->models,
+                       s->n_tris, partial);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kThreads), 0, st, partial, nb, s->n_tris,
+                           pad_origin, d_bounds);
+        e = hipGetLastError();
+    }
+    hipError_t se = hipStreamSynchronize(st);
+    (void)hipFree(partial);
+    if (e != hipSuccess) return hip_fail(ctx, e, "bounds kernels");
+    if (se != hipSuccess) return hip_fail(ctx, se, "bounds sync");
+    return RTQ_OK;
+}
+
+struct BuildTemps {
+    std::vector<void*> ptrs;
+    ~BuildTemps() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t get(T** p, size_t count) {
+        hipError_t e = dev_alloc(p, count);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+}  // namespace
+
+rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
+    hipStream_t st = ctx->stream;
+    const uint32_t n = s->n_tris;
+    const uint32_t nc = 2 * n - 1;
+    free_build(s);
+    s->params = prm;
+    s->n_clusters = nc;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    RTQ_HIP(ctx, hipEventCreate(&ev0));
+    RTQ_HIP(ctx, hipEventCreate(&ev1));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } evg{ev0, ev1};
+    RTQ_HIP(ctx, hipEventRecord(ev0, st));
+
+    // persistent products
+    RTQ_HIP(ctx, dev_alloc(&s->cbox, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->ctri, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->cleft, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->cright, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->cparent, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->ccount, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->sorted_idx, n));
+    RTQ_HIP(ctx, dev_alloc(&s->sorted_codes, n));
+    RTQ_HIP(ctx, dev_alloc(&s->ref_nodes, nc));
+    RTQ_HIP(ctx, dev_alloc(&s->tris, 3ull * n));
+
+    BuildTemps tmp;
+    rtq_aabb* d_bounds = nullptr;
+    uint32_t *tk = nullptr, *tv = nullptr, *c_in = nullptr, *c_out = nullptr, *nn = nullptr;
+    Box6 *b_in = nullptr, *b_out = nullptr;
+    uint2 *flags = nullptr, *offs = nullptr, *tot = nullptr;
+    RTQ_HIP(ctx, tmp.get(&d_bounds, 2));
+    RTQ_HIP(ctx, tmp.get(&tk, n));
+    RTQ_HIP(ctx, tmp.get(&tv, n));
+    RTQ_HIP(ctx, tmp.get(&c_in, n));
+    RTQ_HIP(ctx, tmp.get(&c_out, n));
+    RTQ_HIP(ctx, tmp.get(&nn, n));
+    RTQ_HIP(ctx, tmp.get(&b_in, n));
+    RTQ_HIP(ctx, tmp.get(&b_out, n));
+    RTQ_HIP(ctx, tmp.get(&flags, n));
+    RTQ_HIP(ctx, tmp.get(&offs, n));
+    RTQ_HIP(ctx, tmp.get(&tot, 1));
+    RTQ_HIP(ctx, hipMemsetAsync(nn, 0, sizeof(uint32_t) * n, st));
+
+    // 1. bounds, cube, Morton codes, sort
+    rtq_status rs = compute_bounds(ctx, s, prm.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
+    if (rs != RTQ_OK) return rs;
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, n,
+                       d_bounds, s->sorted_codes, s->sorted_idx);
+    RTQ_HIP(ctx, hipGetLastError());
+    rs = radix_sort_pairs(ctx, s->sorted_codes, s->sorted_idx, tk, tv, n, 30, st);
+    if (rs != RTQ_OK) return rs;
+
+    // 2. PLOC
+    hipLaunchKernelGGL(k_cluster_init, dim3(blocks_for(nc)), dim3(kThreads), 0, st, n, s->cparent,
+                       s->cleft, s->cright, s->ctri);
+    RTQ_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_leaf_init, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models,
+                       s->sorted_idx, n, s->cbox, s->ctri, s->ccount, c_in, b_in);
+    RTQ_HIP(ctx, hipGetLastError());
+    std::vector<std::pair<uint32_t, uint32_t>> pass_ranges;
+    uint32_t iter = n, n_total = n;
+    while (iter > 1) {
+        hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(iter)), dim3(kThreads), 0, st, b_in, iter,
+                           prm.search_radius, nn);
+        RTQ_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_ploc_flags, dim3(blocks_for(iter)), dim3(kThreads), 0, st, nn, iter, flags);
+        RTQ_HIP(ctx, hipGetLastError());
+        rs = scan_u32x2(ctx, flags, offs, iter, tot, st);
+        if (rs != RTQ_OK) return rs;
+        hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(iter)), dim3(kThreads), 0, st, iter, n_total, nn,
+                           flags, offs, c_in, b_in, c_out, b_out, s->cbox, s->cleft, s->cright, s->cparent,
+                           s->ccount, s->ctri);
+        RTQ_HIP(ctx, hipGetLastError());
+        uint2 h_tot;
+        RTQ_HIP(ctx, hipMemcpyAsync(&h_tot, tot, sizeof(uint2), hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        if (h_tot.x == 0)
+            return fail(ctx, RTQ_EINTERNAL, "PLOC pass with %u clusters made no merge", iter);
+        pass_ranges.emplace_back(n_total, h_tot.x);
+        n_total += h_tot.x;
+        iter = h_tot.y;
+        std::swap(c_in, c_out);
+        std::swap(b_in, b_out);
+    }
+    if (n_total != nc) return fail(ctx, RTQ_EINTERNAL, "PLOC produced %u clusters, expected %u", n_total, nc);
+    s->ploc_passes = (uint32_t)pass_ranges.size();
+
+    // 3. pre-order placement, reference node array, leaf-ordered triangles
+    uint32_t *pos = nullptr, *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr,
+             *bidx = nullptr, *ntrav = nullptr;
+    RTQ_HIP(ctx, tmp.get(&pos, nc));
+    RTQ_HIP(ctx, tmp.get(&fl, nc));
+    RTQ_HIP(ctx, tmp.get(&dep, nc));
+    RTQ_HIP(ctx, tmp.get(&maxd, 2));
+    RTQ_HIP(ctx, tmp.get(&flag, nc));
+    RTQ_HIP(ctx, tmp.get(&bidx, nc));
+    RTQ_HIP(ctx, tmp.get(&ntrav, 1));
+    RTQ_HIP(ctx, hipMemsetAsync(maxd, 0, sizeof(uint32_t) * 2, st));
+    const uint32_t root = nc - 1;
+    hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
+    RTQ_HIP(ctx, hipGetLastError());
+    for (auto it = pass_ranges.rbegin(); it != pass_ranges.rend(); ++it) {
+        hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(it->second)), dim3(kThreads), 0, st, it->first,
+                           it->second, s->cleft, s->cright, s->ccount, pos, fl, dep);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_write_ref, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->cbox, s->ctri,
+                       s->cleft, s->cright, pos, fl, dep, s->tri_world, s->ref_nodes, s->tris, maxd);
+    RTQ_HIP(ctx, hipGetLastError());
+
+    // 4. traversal layout: padding from the scene extent
+    rtq_aabb h_bounds[2];
+    uint32_t h_depth = 0;
+    RTQ_HIP(ctx, hipMemcpyAsync(h_bounds, d_bounds, sizeof h_bounds, hipMemcpyDeviceToHost, st));
+    RTQ_HIP(ctx, hipMemcpyAsync(&h_depth, maxd, sizeof h_depth, hipMemcpyDeviceToHost, st));
+    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    float extent = 0.0f;
+    for (int k = 0; k < 3; ++k)
+        extent = std::max(extent, std::max(std::fabs(h_bounds[0].min[k]), std::fabs(h_bounds[0].max[k])));
+    s->box_pad = std::max(std::ldexp(extent, -18), 1e-30f);
+    s->depth = h_depth;
+
+    const uint32_t max_leaf = prm.max_leaf_size;
+    if (n == 1) {
+        s->n_trav = 1;
+        RTQ_HIP(ctx, dev_alloc(&s->nodes, 4));
+        hipLaunchKernelGGL(k_write_single, dim3(1), dim3(1), 0, st, s->cbox, s->box_pad, s->nodes);
+        RTQ_HIP(ctx, hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
+                           s->cleft, s->ccount, pos, flag);
+        RTQ_HIP(ctx, hipGetLastError());
+        rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
+        if (rs != RTQ_OK) return rs;
+        uint32_t h_ntrav = 0;
+        RTQ_HIP(ctx, hipMemcpyAsync(&h_ntrav, ntrav, sizeof h_ntrav, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        s->n_trav = h_ntrav;
+        RTQ_HIP(ctx, dev_alloc(&s->nodes, 4ull * h_ntrav));
+        hipLaunchKernelGGL(k_write_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
+                           s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, s->nodes);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    RTQ_HIP(ctx, hipEventRecord(ev1, st));
+    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    float ms = 0.0f;
+    RTQ_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
+    s->build_ms = ms;
+    s->built = true;
+    return RTQ_OK;
+}
+
+rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_codes) {
+    hipStream_t st = ctx->stream;
+    BuildTemps tmp;
+    rtq_aabb* d_bounds = nullptr;
+    uint32_t* codes = nullptr;
+    RTQ_HIP(ctx, tmp.get(&d_bounds, 2));
+    RTQ_HIP(ctx, tmp.get(&codes, s->n_tris));
+    rtq_status rs = compute_bounds(ctx, s, flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
+    if (rs != RTQ_OK) return rs;
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(s->n_tris)), dim3(kThreads), 0, st, s->tris_ref, s->models,
+                       s->n_tris, d_bounds, codes, (uint32_t*)nullptr);
+    RTQ_HIP(ctx, hipGetLastError());
+    RTQ_HIP(ctx, hipMemcpyAsync(host_codes, codes, sizeof(uint32_t) * s->n_tris, hipMemcpyDeviceToHost, st));
+    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    return RTQ_OK;
+}
+
+}  // namespace rtq

@@ -1,0 +1,110 @@
+// Internal declarations shared by the rtq translation units (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rtq/rtq.h"
+
+namespace rtq {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Child / stack reference of the traversal layout: bit 31 clear = internal node
+// index; bit 31 set = leaf, bits 30..3 first triangle (leaf order), bits 2..0
+// count - 1.  kEmpty ends a traversal.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kMaxTris = (1u << 28) - 16u;
+constexpr uint32_t kMaxLeaf = 8u;
+
+__host__ __device__ inline uint32_t leaf_ref(uint32_t first, uint32_t count) {
+    return kLeafBit | (first << 3) | (count - 1u);
+}
+
+// Model matrix as uploaded: 16 column-major floats + material id (68 B).
+struct DevModel {
+    float m[16];
+    uint32_t material_id;
+};
+static_assert(sizeof(DevModel) == 68, "MeshModelGPU is 68 B");
+
+struct Scene {
+    uint32_t n_tris = 0;
+    uint32_t n_models = 0;
+    rtq_triangle* tris_ref = nullptr;   // N x 64 B, as uploaded
+    DevModel* models = nullptr;         // n_models x 68 B
+    float4* tri_world = nullptr;        // N x 3 float4: world vertices, original order
+
+    // build products
+    bool built = false;
+    rtq_build_params params{};
+    uint32_t n_clusters = 0;            // 2N - 1
+    rtq_aabb* cbox = nullptr;           // cluster boxes
+    uint32_t* ctri = nullptr;           // cluster triangle id (leaves)
+    uint32_t* cleft = nullptr;
+    uint32_t* cright = nullptr;
+    uint32_t* cparent = nullptr;
+    uint32_t* ccount = nullptr;         // leaves below each cluster
+    uint32_t* sorted_idx = nullptr;     // N
+    uint32_t* sorted_codes = nullptr;   // N
+    rtq_bvh_node* ref_nodes = nullptr;  // 2N - 1 pre-order nodes (reference layout)
+    float4* nodes = nullptr;            // traversal nodes, 4 float4 (64 B) each
+    float4* tris = nullptr;             // traversal triangles, leaf order, 3 float4 (48 B) each
+    uint32_t n_trav = 0;
+    uint32_t depth = 0;
+    uint32_t ploc_passes = 0;
+    float box_pad = 0.0f;
+    float build_ms = 0.0f;
+};
+
+}  // namespace rtq
+
+struct rtq_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<uint32_t, rtq::Scene*> scenes;
+    uint32_t next_scene = 1;
+    // trace scratch: per-launch ray counter and the deep part of traversal stacks
+    uint32_t* counter = nullptr;
+    uint32_t* spill = nullptr;
+    size_t spill_words = 0;
+};
+
+namespace rtq {
+
+rtq_status fail(rtq_ctx* ctx, rtq_status s, const char* fmt, ...);
+rtq_status hip_fail(rtq_ctx* ctx, hipError_t e, const char* what);
+Scene* find_scene(rtq_ctx* ctx, uint32_t id);
+hipStream_t pick_stream(rtq_ctx* ctx, void* stream);
+void free_build(Scene* s);
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t count) {
+    return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+}
+
+// Exclusive scan of u32 on device (wrapping); `total` (device, may be null)
+// receives the sum.  Temporary storage is allocated internally.
+rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
+                    uint32_t* total, hipStream_t st);
+// Exclusive scan of pairs of u32 (component-wise).
+rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total,
+                      hipStream_t st);
+// Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
+rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
+                            uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st);
+
+}  // namespace rtq
+
+#define RTQ_HIP(ctx, call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return rtq::hip_fail((ctx), e_, #call); \
+    } while (0)

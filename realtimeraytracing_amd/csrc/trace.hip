@@ -1,0 +1,375 @@
+// BVH traversal + ray/triangle intersection on gfx950.
+//
+// Replaces getClosestHitBVH / intersectBVH / rayTriangleIntersection
+// (raytracer.glsl:102-147, 182-295).  Design:
+//  - persistent waves: a wave refills its idle lanes from a global ray counter
+//    (one atomic per refill, lanes ranked with mbcnt over the ballot) whenever
+//    at least `refill` lanes are idle, so finished lanes do not wait for the
+//    slowest ray of the wave;
+//  - two-child 64-B nodes (both child boxes in one 64-B fetch), nearer child
+//    first, farther child pushed; traversal stack of child refs in LDS for the
+//    top kStackLds entries ([slot][lane] so lanes hit distinct banks), deeper
+//    entries in a per-thread global spill column sized from the tree depth;
+//  - conservative slab test: boxes are stored widened outward by a pad of
+//    2^-18 of the scene extent, and the interval is [tmin, t_best] inclusive so
+//    a later, lower-id triangle at the same t is still reached;
+//  - leaf test: watertight FP32 (Woop, Benthin & Wald, JCGT 2013) with the FP64
+//    re-evaluation of edge functions that are exactly zero, or the reference's
+//    Moller-Trumbore (RefMT); closest hit keeps the smallest (t, id).
+// Float expressions of the leaf tests are evaluated in the order of the CPU
+// restatement (built with -ffp-contract=off), so t/u/v are bit-identical.
+#include <algorithm>
+
+#include "rtq_internal.hpp"
+
+namespace rtq {
+namespace {
+
+constexpr int kTraceThreads = 256;
+constexpr uint32_t kStackLds = 16;
+
+struct TraceArgs {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    const rtq_ray* __restrict__ rays;
+    rtq_hit* __restrict__ hits;
+    uint32_t n;
+    uint32_t* counter;
+    uint32_t* spill;
+    uint32_t spill_stride;
+    uint32_t refill;
+    uint32_t* stat_nodes;
+    uint32_t* stat_tris;
+};
+
+__device__ __forceinline__ float sel3(float4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ float sel3f(float x, float y, float z, int k) {
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
+// Watertight test; on a hit with tmin <= t <= tmax returns true and t, V, W, det.
+__device__ __forceinline__ bool wt_test(float4 p0, float4 p1, float4 p2, int kx, int ky, int kz, float opx,
+                                        float opy, float opz, float sx, float sy, float sz, float tmin,
+                                        float tmax, float& t, float& V_out, float& W_out, float& det_out) {
+    const float Akx = sel3(p0, kx) - opx, Aky = sel3(p0, ky) - opy, Akz = sel3(p0, kz) - opz;
+    const float Bkx = sel3(p1, kx) - opx, Bky = sel3(p1, ky) - opy, Bkz = sel3(p1, kz) - opz;
+    const float Ckx = sel3(p2, kx) - opx, Cky = sel3(p2, ky) - opy, Ckz = sel3(p2, kz) - opz;
+    const float Ax = Akx - sx * Akz;
+    const float Ay = Aky - sy * Akz;
+    const float Bx = Bkx - sx * Bkz;
+    const float By = Bky - sy * Bkz;
+    const float Cx = Ckx - sx * Ckz;
+    const float Cy = Cky - sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        const double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        const double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        const double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = (U + V) + W;
+    if (det == 0.0f) return false;
+    const float Az = sz * Akz;
+    const float Bz = sz * Bkz;
+    const float Cz = sz * Ckz;
+    const float T = (U * Az + V * Bz) + W * Cz;
+    const float tt = T / det;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt;
+    V_out = V;
+    W_out = W;
+    det_out = det;
+    return true;
+}
+
+// raytracer.glsl:102-147 (shader vertex order: its p1 is P2, its p2 is P1).
+__device__ __forceinline__ bool refmt_test(float4 q0, float4 q1, float4 q2, float ox, float oy, float oz,
+                                           float dx, float dy, float dz, float tmin, float tmax, float& t,
+                                           float& b0_out, float& b1_out) {
+    const float p0x = q0.x, p0y = q0.y, p0z = q0.z;
+    const float e0x = q2.x - p0x, e0y = q2.y - p0y, e0z = q2.z - p0z;  // shader p1 - p0
+    const float e1x = q1.x - p0x, e1y = q1.y - p0y, e1z = q1.z - p0z;  // shader p2 - p0
+    // n = cross(e1, e0)
+    const float nx = e1y * e0z - e0y * e1z;
+    const float ny = e1z * e0x - e0z * e1x;
+    const float nz = e1x * e0y - e0x * e1y;
+    // q = cross(d, e1)
+    const float qx = dy * e1z - e1y * dz;
+    const float qy = dz * e1x - e1z * dx;
+    const float qz = dx * e1y - e1x * dy;
+    const float a = (e0x * qx + e0y * qy) + e0z * qz;
+    if ((nx * dx + ny * dy) + nz * dz >= 0.0f || fabsf(a) < 1e-4f) return false;
+    const float sx = (ox - p0x) / a, sy = (oy - p0y) / a, sz = (oz - p0z) / a;
+    // r = cross(s, e0)
+    const float rx = sy * e0z - e0y * sz;
+    const float ry = sz * e0x - e0z * sx;
+    const float rz = sx * e0y - e0x * sy;
+    const float b0 = (sx * qx + sy * qy) + sz * qz;
+    const float b1 = (rx * dx + ry * dy) + rz * dz;
+    const float b2 = (1.0f - b0) - b1;
+    if (b0 < 0.0f || b1 < 0.0f || b2 < 0.0f) return false;
+    const float tt = (e1x * rx + e1y * ry) + e1z * rz;
+    if (tt < 0.0f) return false;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt;
+    b0_out = b0;
+    b1_out = b1;
+    return true;
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+template <int LEAF, bool ANY, bool STATS>
+__global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
+    __shared__ uint32_t s_stack[kStackLds][kTraceThreads];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    uint32_t* const spill = a.spill + (size_t)blockIdx.x * kTraceThreads + tid;
+    const size_t stride = a.spill_stride;
+
+    int rid = -1;
+    bool exhausted = false;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
+    float ix = 0, iy = 0, iz = 0, oix = 0, oiy = 0, oiz = 0;
+    float tmin = 0, tbest = 0;
+    int kx = 0, ky = 1, kz = 2;
+    float sx = 0, sy = 0, sz = 0, opx = 0, opy = 0, opz = 0;
+    int best_id = -1;
+    float best_t = 0, best_a = 0, best_b = 0, best_c = 0;
+    uint32_t cur = kEmpty, sp = 0;
+    uint32_t n_nodes = 0, n_tests = 0;
+
+    auto push = [&](uint32_t v) {
+        if (sp < kStackLds) s_stack[sp][tid] = v;
+        else spill[(size_t)(sp - kStackLds) * stride] = v;
+        ++sp;
+    };
+    auto pop = [&]() -> uint32_t {
+        if (sp == 0) return kEmpty;
+        --sp;
+        return sp < kStackLds ? s_stack[sp][tid] : spill[(size_t)(sp - kStackLds) * stride];
+    };
+    auto finish = [&]() {
+        rtq_hit h;
+        if (best_id >= 0) {
+            h.id = best_id;
+            h.t = best_t;
+            if (LEAF == 0) {
+                h.u = best_a / best_c;
+                h.v = best_b / best_c;
+            } else {
+                h.u = best_a;
+                h.v = best_b;
+            }
+        } else {
+            h.id = -1;
+            h.t = INFINITY;
+            h.u = 0.0f;
+            h.v = 0.0f;
+        }
+        a.hits[rid] = h;
+        if (STATS) {
+            if (a.stat_nodes) a.stat_nodes[rid] = n_nodes;
+            if (a.stat_tris) a.stat_tris[rid] = n_tests;
+        }
+        rid = -1;
+    };
+
+    for (;;) {
+        // ---- refill idle lanes from the global ray counter
+        const bool need = (rid < 0) && !exhausted;
+        const uint64_t need_mask = __ballot(need);
+        if (need_mask) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need_mask) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(a.counter, (uint32_t)__popcll(need_mask));
+            base = __shfl(base, (int)leader);
+            if (need) {
+                const uint32_t r = base + __builtin_amdgcn_mbcnt_hi(
+                                              (uint32_t)(need_mask >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)need_mask, 0u));
+                if (r < a.n) {
+                    rid = (int)r;
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays + r);
+                    const float4 r0 = rp[0], r1 = rp[1];
+                    ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
+                    dx = r1.x; dy = r1.y; dz = r1.z; tbest = r1.w;
+                    best_id = -1;
+                    n_nodes = 0;
+                    n_tests = 0;
+                    sp = 0;
+                    cur = 0;
+                    // box-test reciprocals (zero components replaced by +-1e-30)
+                    const float ex = fabsf(dx) < 1e-30f ? copysignf(1e-30f, dx) : dx;
+                    const float ey = fabsf(dy) < 1e-30f ? copysignf(1e-30f, dy) : dy;
+                    const float ez = fabsf(dz) < 1e-30f ? copysignf(1e-30f, dz) : dz;
+                    ix = __builtin_amdgcn_rcpf(ex);
+                    iy = __builtin_amdgcn_rcpf(ey);
+                    iz = __builtin_amdgcn_rcpf(ez);
+                    oix = ox * ix; oiy = oy * iy; oiz = oz * iz;
+                    if (LEAF == 0) {
+                        // watertight ray set-up: kz = first axis of max |d|
+                        const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+                        int z = 0;
+                        float m = ax;
+                        if (ay > m) { z = 1; m = ay; }
+                        if (az > m) { z = 2; }
+                        int x = z + 1; if (x == 3) x = 0;
+                        int y = x + 1; if (y == 3) y = 0;
+                        const float dkz = sel3f(dx, dy, dz, z);
+                        if (dkz < 0.0f) { const int tmp = x; x = y; y = tmp; }
+                        kx = x; ky = y; kz = z;
+                        sx = sel3f(dx, dy, dz, x) / dkz;
+                        sy = sel3f(dx, dy, dz, y) / dkz;
+                        sz = 1.0f / dkz;
+                        opx = sel3f(ox, oy, oz, x);
+                        opy = sel3f(ox, oy, oz, y);
+                        opz = sel3f(ox, oy, oz, z);
+                        if (dkz == 0.0f) cur = kEmpty;  // zero direction: nothing can be hit
+                    }
+                    if (!(tmin <= tbest)) cur = kEmpty;  // empty interval
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(rid >= 0) == 0) break;
+
+        // ---- traversal steps until enough lanes are idle
+        for (;;) {
+            if (rid >= 0) {
+                // descend through internal nodes
+                while (cur != kEmpty && !(cur & kLeafBit)) {
+                    const float4* nd = a.nodes + 4ull * cur;
+                    const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                    if (STATS) ++n_nodes;
+                    const float ax0 = __builtin_fmaf(n0.x, ix, -oix), bx0 = __builtin_fmaf(n0.y, ix, -oix);
+                    const float ay0 = __builtin_fmaf(n0.z, iy, -oiy), by0 = __builtin_fmaf(n0.w, iy, -oiy);
+                    const float az0 = __builtin_fmaf(n1.x, iz, -oiz), bz0 = __builtin_fmaf(n1.y, iz, -oiz);
+                    const float ax1 = __builtin_fmaf(n1.z, ix, -oix), bx1 = __builtin_fmaf(n1.w, ix, -oix);
+                    const float ay1 = __builtin_fmaf(n2.x, iy, -oiy), by1 = __builtin_fmaf(n2.y, iy, -oiy);
+                    const float az1 = __builtin_fmaf(n2.z, iz, -oiz), bz1 = __builtin_fmaf(n2.w, iz, -oiz);
+                    const float tn0 = fmaxf(fmaxf(fminf(ax0, bx0), fminf(ay0, by0)), fmaxf(fminf(az0, bz0), tmin));
+                    const float tf0 = fminf(fminf(fmaxf(ax0, bx0), fmaxf(ay0, by0)), fminf(fmaxf(az0, bz0), tbest));
+                    const float tn1 = fmaxf(fmaxf(fminf(ax1, bx1), fminf(ay1, by1)), fmaxf(fminf(az1, bz1), tmin));
+                    const float tf1 = fminf(fminf(fmaxf(ax1, bx1), fmaxf(ay1, by1)), fminf(fmaxf(az1, bz1), tbest));
+                    const bool h0 = tn0 <= tf0, h1 = tn1 <= tf1;
+                    const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
+                    if (h0 && h1) {
+                        const bool first0 = tn0 <= tn1;
+                        push(first0 ? r1 : r0);
+                        cur = first0 ? r0 : r1;
+                    } else if (h0) {
+                        cur = r0;
+                    } else if (h1) {
+                        cur = r1;
+                    } else {
+                        cur = pop();
+                    }
+                }
+                // one leaf
+                if (cur != kEmpty) {
+                    const uint32_t first = (cur >> 3) & 0x0FFFFFFFu;
+                    const uint32_t count = (cur & 7u) + 1u;
+                    bool done = false;
+                    for (uint32_t k = 0; k < count; ++k) {
+                        const uint32_t ti = first + k;
+                        const float4* tp = a.tris + 3ull * ti;
+                        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                        if (STATS) ++n_tests;
+                        const int id = (int)__float_as_uint(p0.w);
+                        float t, ua, ub, uc = 0.0f;
+                        bool h;
+                        if (LEAF == 0)
+                            h = wt_test(p0, p1, p2, kx, ky, kz, opx, opy, opz, sx, sy, sz, tmin, tbest, t, ua, ub, uc);
+                        else
+                            h = refmt_test(p0, p1, p2, ox, oy, oz, dx, dy, dz, tmin, tbest, t, ua, ub);
+                        if (h && (best_id < 0 || t < best_t || (t == best_t && id < best_id))) {
+                            best_id = id;
+                            best_t = t;
+                            best_a = ua;
+                            best_b = ub;
+                            best_c = uc;
+                            tbest = t;
+                            if (ANY) { done = true; break; }
+                        }
+                    }
+                    cur = done ? kEmpty : pop();
+                }
+                if (cur == kEmpty) finish();
+            }
+            const uint64_t idle = __ballot(rid < 0 && !exhausted);
+            const uint64_t active = __ballot(rid >= 0);
+            if (active == 0 || (uint32_t)__popcll(idle) >= a.refill) break;
+        }
+    }
+}
+
+template <int LEAF, bool ANY, bool STATS>
+rtq_status launch(rtq_ctx* ctx, const Scene* s, const TraceArgs& base, uint32_t n, hipStream_t st) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, ANY, STATS>,
+                                                               kTraceThreads, 0);
+    if (e != hipSuccess || per_cu <= 0) per_cu = 4;
+    uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
+                                         (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
+    if (blocks == 0) blocks = 1;
+    const uint64_t threads = blocks * kTraceThreads;
+    // stack entries needed: one per tree level below the root, plus one
+    const uint64_t depth_entries = (uint64_t)s->depth + 2;
+    const uint64_t spill_words = depth_entries > kStackLds ? (depth_entries - kStackLds) * threads : 1;
+    if (spill_words > ctx->spill_words) {
+        if (ctx->spill) {
+            RTQ_HIP(ctx, hipStreamSynchronize(st));
+            (void)hipFree(ctx->spill);
+            ctx->spill = nullptr;
+            ctx->spill_words = 0;
+        }
+        RTQ_HIP(ctx, dev_alloc(&ctx->spill, spill_words));
+        ctx->spill_words = spill_words;
+    }
+    TraceArgs args = base;
+    args.counter = ctx->counter;
+    args.spill = ctx->spill;
+    args.spill_stride = (uint32_t)threads;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, ctx->counter);
+    RTQ_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL((k_trace<LEAF, ANY, STATS>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st, args);
+    RTQ_HIP(ctx, hipGetLastError());
+    return RTQ_OK;
+}
+
+}  // namespace
+
+uint32_t g_refill_threshold = 16;
+
+rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,
+                 uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st) {
+    if (n == 0) return RTQ_OK;
+    if (n >= 0xFFFFFFFFull) return fail(ctx, RTQ_ECAPACITY, "trace: n_rays must be < 2^32 - 1");
+    TraceArgs args{};
+    args.nodes = s->nodes;
+    args.tris = s->tris;
+    args.rays = rays;
+    args.hits = hits;
+    args.n = (uint32_t)n;
+    args.refill = g_refill_threshold;
+    args.stat_nodes = stat_nodes;
+    args.stat_tris = stat_tris;
+    const bool any = flags & RTQ_TRACE_ANY;
+    const bool refmt = flags & RTQ_TRACE_LEAF_REFMT;
+    const uint32_t nn = (uint32_t)n;
+    if (!stats) {
+        if (!refmt) return any ? launch<0, true, false>(ctx, s, args, nn, st) : launch<0, false, false>(ctx, s, args, nn, st);
+        return any ? launch<1, true, false>(ctx, s, args, nn, st) : launch<1, false, false>(ctx, s, args, nn, st);
+    }
+    if (!refmt) return any ? launch<0, true, true>(ctx, s, args, nn, st) : launch<0, false, true>(ctx, s, args, nn, st);
+    return any ? launch<1, true, true>(ctx, s, args, nn, st) : launch<1, false, true>(ctx, s, args, nn, st);
+}
+
+}  // namespace rtq

@@ -1,0 +1,239 @@
+"""ctypes bindings of librtq (include/rtq/rtq.h).
+
+Python is the test and benchmark harness; the product is the C ABI.  Loading
+the library never touches the GPU; the first rtq_create does.  If librtq.so
+is missing this module raises: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "librtq.so"
+
+# ---- record layouts (include/rtq/scene_types.h) -------------------------
+TRIANGLE_DTYPE = np.dtype({"names": ["p0", "p1", "p2", "model_id", "pad"],
+                           "formats": [("<f4", 4), ("<f4", 4), ("<f4", 4), "<u4", ("<u4", 3)],
+                           "offsets": [0, 16, 32, 48, 52], "itemsize": 64})
+MODEL_DTYPE = np.dtype({"names": ["m", "material_id"], "formats": [("<f4", 16), "<u4"],
+                        "offsets": [0, 64], "itemsize": 68})
+AABB_DTYPE = np.dtype({"names": ["min", "pad0", "max", "pad1"],
+                       "formats": [("<f4", 3), "<f4", ("<f4", 3), "<f4"],
+                       "offsets": [0, 12, 16, 28], "itemsize": 32})
+NODE_DTYPE = np.dtype({"names": ["box", "triangle_id", "left_child", "right_child", "pad"],
+                       "formats": [AABB_DTYPE, "<u4", "<u4", "<u4", "<u4"],
+                       "offsets": [0, 32, 36, 40, 44], "itemsize": 48})
+RAY_DTYPE = np.dtype({"names": ["o", "tmin", "d", "tmax"], "formats": [("<f4", 3), "<f4", ("<f4", 3), "<f4"],
+                      "offsets": [0, 12, 16, 28], "itemsize": 32})
+HIT_DTYPE = np.dtype({"names": ["id", "t", "u", "v"], "formats": ["<i4", "<f4", "<f4", "<f4"],
+                      "offsets": [0, 4, 8, 12], "itemsize": 16})
+
+RTQ_OK, RTQ_EINVAL, RTQ_ENOMEM, RTQ_EHIP, RTQ_ECAPACITY, RTQ_EINTERNAL, RTQ_ENOTBUILT = range(7)
+RTQ_BUILD_REF_PADDED_BOUNDS = 0x1
+RTQ_TRACE_CLOSEST = 0x0
+RTQ_TRACE_ANY = 0x1
+RTQ_TRACE_LEAF_REFMT = 0x2
+
+EXPORTED = [
+    "rtq_abi_version", "rtq_status_string", "rtq_create", "rtq_destroy", "rtq_last_error",
+    "rtq_synchronize", "rtq_scene_upload", "rtq_scene_release", "rtq_build_params_default",
+    "rtq_bvh_build", "rtq_bvh_info_get", "rtq_bvh_export_ref", "rtq_bvh_export_clusters",
+    "rtq_morton_codes", "rtq_sort_pairs", "rtq_scan_exclusive_u32", "rtq_trace", "rtq_trace_stats",
+    "rtq_trace_host", "rtq_raygen_pinhole", "rtq_raygen_diffuse", "rtq_raygen_shadow",
+]
+
+
+class BuildParams(ctypes.Structure):
+    _fields_ = [("search_radius", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("n_tris", ctypes.c_uint32), ("n_ref_nodes", ctypes.c_uint32),
+                ("n_trav_nodes", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("ploc_iterations", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
+                ("box_pad", ctypes.c_float), ("build_ms", ctypes.c_float)]
+
+
+class Pinhole(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("forward", ctypes.c_float * 3), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
+class RtqError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"rtq status {status}: {message}")
+        self.status = status
+
+
+_lib: ctypes.CDLL | None = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load librtq.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(str(p))
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    st = ctypes.c_int
+    sigs = {
+        "rtq_abi_version": (i32, []),
+        "rtq_status_string": (ctypes.c_char_p, [st]),
+        "rtq_create": (st, [ctypes.POINTER(vp), i32]),
+        "rtq_destroy": (None, [vp]),
+        "rtq_last_error": (ctypes.c_char_p, [vp]),
+        "rtq_synchronize": (st, [vp]),
+        "rtq_scene_upload": (st, [vp, vp, u32, vp, u32, ctypes.POINTER(u32)]),
+        "rtq_scene_release": (st, [vp, u32]),
+        "rtq_build_params_default": (None, [ctypes.POINTER(BuildParams)]),
+        "rtq_bvh_build": (st, [vp, u32, ctypes.POINTER(BuildParams)]),
+        "rtq_bvh_info_get": (st, [vp, u32, ctypes.POINTER(BvhInfo)]),
+        "rtq_bvh_export_ref": (st, [vp, u32, vp, u64, ctypes.POINTER(u64)]),
+        "rtq_bvh_export_clusters": (st, [vp, u32, u64, vp, vp, vp, vp, vp, vp, vp]),
+        "rtq_morton_codes": (st, [vp, u32, u32, vp]),
+        "rtq_sort_pairs": (st, [vp, vp, vp, u64, u32, vp]),
+        "rtq_scan_exclusive_u32": (st, [vp, vp, vp, u64, vp]),
+        "rtq_trace": (st, [vp, u32, vp, u64, u32, vp, vp]),
+        "rtq_trace_stats": (st, [vp, u32, vp, u64, u32, vp, vp, vp, vp]),
+        "rtq_trace_host": (st, [vp, u32, vp, u64, u32, vp]),
+        "rtq_raygen_pinhole": (st, [vp, ctypes.POINTER(Pinhole), vp, vp]),
+        "rtq_raygen_diffuse": (st, [vp, u32, vp, vp, u64, u64, ctypes.c_float, vp, vp]),
+        "rtq_raygen_shadow": (st, [vp, u32, vp, vp, u64, vp, u32, ctypes.c_float, vp, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One device + stream (rtq_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        s = self.lib.rtq_create(ctypes.byref(h), device)
+        if s != RTQ_OK:
+            raise RtqError(s, f"rtq_create(device={device}) failed: "
+                              f"{self.lib.rtq_status_string(s).decode()}")
+        self.h = h
+
+    def check(self, s: int) -> None:
+        if s != RTQ_OK:
+            raise RtqError(s, self.lib.rtq_last_error(self.h).decode())
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.rtq_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self) -> None:
+        self.check(self.lib.rtq_synchronize(self.h))
+
+    def upload(self, tris: np.ndarray, models: np.ndarray) -> "Scene":
+        tris = np.ascontiguousarray(tris, dtype=TRIANGLE_DTYPE)
+        models = np.ascontiguousarray(models, dtype=MODEL_DTYPE)
+        sid = ctypes.c_uint32()
+        self.check(self.lib.rtq_scene_upload(self.h, _ptr(tris), len(tris), _ptr(models), len(models),
+                                             ctypes.byref(sid)))
+        return Scene(self, sid.value, len(tris))
+
+    # device-pointer primitives
+    def sort_pairs(self, keys_ptr: int, vals_ptr: int, n: int, key_bits: int = 32, stream: int | None = None):
+        self.check(self.lib.rtq_sort_pairs(self.h, keys_ptr, vals_ptr, n, key_bits, stream))
+
+    def scan_exclusive(self, in_ptr: int, out_ptr: int, n: int, stream: int | None = None):
+        self.check(self.lib.rtq_scan_exclusive_u32(self.h, in_ptr, out_ptr, n, stream))
+
+    def raygen_pinhole(self, cam: Pinhole, rays_ptr: int, stream: int | None = None):
+        self.check(self.lib.rtq_raygen_pinhole(self.h, ctypes.byref(cam), rays_ptr, stream))
+
+
+class Scene:
+    def __init__(self, ctx: Context, sid: int, n_tris: int):
+        self.ctx, self.id, self.n_tris = ctx, sid, n_tris
+
+    def release(self) -> None:
+        self.ctx.check(self.ctx.lib.rtq_scene_release(self.ctx.h, self.id))
+
+    def build(self, search_radius: int = 16, max_leaf_size: int = 4, flags: int = 0) -> "BvhInfo":
+        p = BuildParams(search_radius, max_leaf_size, flags, 0)
+        self.ctx.check(self.ctx.lib.rtq_bvh_build(self.ctx.h, self.id, ctypes.byref(p)))
+        return self.info()
+
+    def info(self) -> BvhInfo:
+        out = BvhInfo()
+        self.ctx.check(self.ctx.lib.rtq_bvh_info_get(self.ctx.h, self.id, ctypes.byref(out)))
+        return out
+
+    def export_ref(self) -> np.ndarray:
+        n = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_ref(self.ctx.h, self.id, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=NODE_DTYPE)
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_ref(self.ctx.h, self.id, _ptr(out), n.value, ctypes.byref(n)))
+        return out
+
+    def export_clusters(self) -> dict:
+        nc = 2 * self.n_tris - 1
+        out = {
+            "boxes": np.zeros(nc, dtype=AABB_DTYPE),
+            "triangle_id": np.zeros(nc, np.uint32), "left": np.zeros(nc, np.uint32),
+            "right": np.zeros(nc, np.uint32), "parent": np.zeros(nc, np.uint32),
+            "sorted_indices": np.zeros(self.n_tris, np.uint32), "sorted_codes": np.zeros(self.n_tris, np.uint32),
+        }
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_clusters(
+            self.ctx.h, self.id, nc, *(_ptr(out[k]) for k in
+                                       ("boxes", "triangle_id", "left", "right", "parent",
+                                        "sorted_indices", "sorted_codes"))))
+        return out
+
+    def morton_codes(self, flags: int = 0) -> np.ndarray:
+        out = np.zeros(self.n_tris, np.uint32)
+        self.ctx.check(self.ctx.lib.rtq_morton_codes(self.ctx.h, self.id, flags, _ptr(out)))
+        return out
+
+    def trace(self, rays_ptr: int, n: int, hits_ptr: int, flags: int = 0, stream: int | None = None) -> None:
+        self.ctx.check(self.ctx.lib.rtq_trace(self.ctx.h, self.id, rays_ptr, n, flags, hits_ptr, stream))
+
+    def trace_stats(self, rays_ptr: int, n: int, hits_ptr: int, nodes_ptr: int | None, tris_ptr: int | None,
+                    flags: int = 0, stream: int | None = None) -> None:
+        self.ctx.check(self.ctx.lib.rtq_trace_stats(self.ctx.h, self.id, rays_ptr, n, flags, hits_ptr,
+                                                    nodes_ptr, tris_ptr, stream))
+
+    def trace_host(self, rays: np.ndarray, flags: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        self.ctx.check(self.ctx.lib.rtq_trace_host(self.ctx.h, self.id, _ptr(rays), len(rays), flags, _ptr(hits)))
+        return hits
+
+    def raygen_diffuse(self, prim_ptr: int, hits_ptr: int, n: int, seed: int, eps: float, out_ptr: int,
+                       stream: int | None = None) -> None:
+        self.ctx.check(self.ctx.lib.rtq_raygen_diffuse(self.ctx.h, self.id, prim_ptr, hits_ptr, n, seed,
+                                                       ctypes.c_float(eps), out_ptr, stream))
+
+    def raygen_shadow(self, prim_ptr: int, hits_ptr: int, n: int, lights: np.ndarray, eps: float, out_ptr: int,
+                      stream: int | None = None) -> None:
+        lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1)
+        self.ctx.check(self.ctx.lib.rtq_raygen_shadow(self.ctx.h, self.id, prim_ptr, hits_ptr, n, _ptr(lights),
+                                                      len(lights) // 3, ctypes.c_float(eps), out_ptr, stream))

@@ -153,7 +153,9 @@ rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t
 
 /* Trace n_rays rays against the scene's BVH.  Replaces the traversal of
  * getClosestHitBVH (raytracer.glsl:246-295) dispatched by drawOneFrame
- * (application.cpp:220-246).  Device buffers, asynchronous on `stream`. */
+ * (application.cpp:220-246).  Device buffers, asynchronous on `stream`; one
+ * kernel launch.  Trace calls of one context share its work counter, so calls
+ * issued on different streams must not run concurrently. */
 rtq_status rtq_trace(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays,
                      uint32_t flags, rtq_hit* hits_dev, void* stream);
 

@@ -258,8 +258,11 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
     ctx->device = hip_device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+    // counter[0]: next ray of a persistent trace launch, counter[1]: blocks done;
+    // zero here, then reset by the last block of every launch
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&ctx->counter), 256) != hipSuccess) {
+        hipMalloc(reinterpret_cast<void**>(&ctx->counter), 256) != hipSuccess ||
+        hipMemset(ctx->counter, 0, 256) != hipSuccess) {
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return RTQ_EHIP;

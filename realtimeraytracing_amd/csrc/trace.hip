@@ -122,8 +122,6 @@ __device__ __forceinline__ bool refmt_test(float4 q0, float4 q1, float4 q2, floa
     return true;
 }
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
-
 template <int LEAF, bool ANY, bool STATS>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     __shared__ uint32_t s_stack[kStackLds][kTraceThreads];
@@ -308,6 +306,18 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
             if (active == 0 || (uint32_t)__popcll(idle) >= a.refill) break;
         }
     }
+    // A wave leaves the loop only after its last fetch ran past n, so when every
+    // block has arrived here no fetch is outstanding: the last block resets the
+    // ray counter (counter[0]) and the arrival counter (counter[1]) for the next
+    // launch on the stream.
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(a.counter + 1, 1u) == gridDim.x - 1) {
+            atomicExch(a.counter, 0u);
+            atomicExch(a.counter + 1, 0u);
+        }
+    }
 }
 
 template <int LEAF, bool ANY, bool STATS>
@@ -337,8 +347,6 @@ rtq_status launch(rtq_ctx* ctx, const Scene* s, const TraceArgs& base, uint32_t 
     args.counter = ctx->counter;
     args.spill = ctx->spill;
     args.spill_stride = (uint32_t)threads;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, ctx->counter);
-    RTQ_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL((k_trace<LEAF, ANY, STATS>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st, args);
     RTQ_HIP(ctx, hipGetLastError());
     return RTQ_OK;

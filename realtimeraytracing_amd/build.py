@@ -85,7 +85,24 @@ def build_oracle(verbose: bool = False) -> Path:
     return out
 
 
+def build_cpp_check(verbose: bool = False) -> Path:
+    """The C++ wrapper check program run by the GPU tests (tests/cpp)."""
+    src = ROOT / "tests" / "cpp" / "wrapper_check.cpp"
+    out_dir = ROOT / "tests" / "cpp" / "build"
+    out_dir.mkdir(exist_ok=True)
+    exe = out_dir / "wrapper_check"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+           f"-L{PKG}", "-lrtq", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../../realtimeraytracing_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"wrapper_check build failed:\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
     if "--with-oracle" in sys.argv:
         build_oracle(verbose=True)
+        build_cpp_check(verbose=True)

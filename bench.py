@@ -10,10 +10,16 @@ the GPU.  One step = trace all diffuse rays + trace all shadow rays, inputs
 resident in HBM.  Mrays/s counts only rays with a non-empty interval (every
 primary ray hits, so all 10,368,000 are live), M = 10^6.
 
+Forms measured on the same scene and rays (hit records are bit-identical
+between them, checked in tests/test_gpu_wide.py):
+  plain       binary 64-B nodes
+  compressed  8-wide 128-B nodes with 8-bit outward-rounded child boxes
+`value` is the fastest form; `detail.forms` has every form.
+
 Prints one JSON line with the metric, a roofline object for the dominant
-trace kernel (algorithmic bytes per launch from the traversal counters over
-its HIP-event duration) and the CPU baseline (the reference traversal
-restated in C, run on this host's cores over the same rays).
+trace kernel of that form (algorithmic bytes per launch from the traversal
+counters over its HIP-event duration) and the CPU baseline (the reference
+traversal restated in C, run on this host's cores over the same rays).
 """
 from __future__ import annotations
 
@@ -30,10 +36,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64      # two-child traversal node
 TRI_BYTES = 48       # leaf triangle record
 RAY_BYTES = 32
 HIT_BYTES = 16
+FORMS = {"plain": 64, "compressed": 128}   # node bytes per visit
 
 
 def parse():
@@ -41,19 +47,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--forms", default="plain,compressed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_shadow_anyhit.txt"))
+    ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     return ap.parse_args()
 
 
-def read_traffic(path: str):
-    """HBM bytes per launch of the dominant kernel, from the committed rocprof
-    counter summary (line `traffic_bytes_per_launch: <value>`), else None."""
+def read_traffic(path: str, kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprof counter
+    summary (lines `traffic_bytes_per_launch <kernel>: <value>`), else None."""
     p = Path(path)
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -65,7 +72,8 @@ def main() -> int:
                                    "the trace path shards by ray ranges with a replicated BVH (see DESIGN.md)"}))
         return 2
     import torch
-    from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_ANY, Context, devmem, scenes
+    from realtimeraytracing_amd import (HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_ANY, RTQ_TRACE_BINARY, Context, devmem,
+                                        scenes)
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -80,12 +88,12 @@ def main() -> int:
     tris, models = scenes.terrain_scene(1024, seed=0)
     ctx = Context(0)
     sc = ctx.upload(tris, models)
-    info = sc.build(16, 4, 0)
+    info = sc.build(16, 4, 0, node_width=8)     # builds the binary and the 8-wide layouts
     cam = scenes.pinhole(**scenes.TERRAIN_CAMERA)
     n = cam.width * cam.height
     d_prim, d_phit = devmem.empty(n, RAY_DTYPE), devmem.empty(n, HIT_DTYPE)
     ctx.raygen_pinhole(cam, d_prim.data_ptr(), sp)
-    sc.trace(d_prim.data_ptr(), n, d_phit.data_ptr(), 0, sp)
+    sc.trace(d_prim.data_ptr(), n, d_phit.data_ptr(), RTQ_TRACE_BINARY, sp)
     d_diff, d_shad = devmem.empty(n, RAY_DTYPE), devmem.empty(4 * n, RAY_DTYPE)
     sc.raygen_diffuse(d_prim.data_ptr(), d_phit.data_ptr(), n, scenes.DIFFUSE_SEED, scenes.SECONDARY_EPS,
                       d_diff.data_ptr(), sp)
@@ -99,53 +107,64 @@ def main() -> int:
     live_rays = n_live + 4 * n_live
     setup_s = time.time() - t0
 
-    # traversal counters (outside the timed region): algorithmic bytes per launch
-    counters = {}
-    for name, d_rays, cnt, d_hits, flags in (("diffuse", d_diff, n_diff, d_hd, 0),
-                                              ("shadow", d_shad, n_shad, d_hs, RTQ_TRACE_ANY)):
-        dv, dt = devmem.empty(cnt, np.uint32), devmem.empty(cnt, np.uint32)
-        sc.trace_stats(d_rays.data_ptr(), cnt, d_hits.data_ptr(), dv.data_ptr(), dt.data_ptr(), flags, sp)
+    results = {}
+    for form in args.forms.split(","):
+        layout = RTQ_TRACE_BINARY if form == "plain" else 0
+        node_bytes = FORMS[form]
+        # traversal counters (outside the timed region): algorithmic bytes per launch
+        counters = {}
+        for name, d_rays, cnt, d_hits, flags in (("diffuse", d_diff, n_diff, d_hd, 0),
+                                                  ("shadow", d_shad, n_shad, d_hs, RTQ_TRACE_ANY)):
+            dv, dt = devmem.empty(cnt, np.uint32), devmem.empty(cnt, np.uint32)
+            sc.trace_stats(d_rays.data_ptr(), cnt, d_hits.data_ptr(), dv.data_ptr(), dt.data_ptr(),
+                           flags | layout, sp)
+            torch.cuda.synchronize()
+            v = int(devmem.to_host(dv, np.uint32).astype(np.int64).sum())
+            t = int(devmem.to_host(dt, np.uint32).astype(np.int64).sum())
+            counters[name] = dict(rays=cnt, node_visits=v, tri_tests=t,
+                                  bytes=cnt * (RAY_BYTES + HIT_BYTES) + v * node_bytes + t * TRI_BYTES)
+
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            sc.trace(d_diff.data_ptr(), n_diff, d_hd.data_ptr(), layout, sp)
+            if ev is not None:
+                ev[1].record(stream)
+            sc.trace(d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY | layout, sp)
+            if ev is not None:
+                ev[2].record(stream)
+
+        for _ in range(args.warmup):
+            step()
         torch.cuda.synchronize()
-        v = devmem.to_host(dv, np.uint32).astype(np.int64)
-        t = devmem.to_host(dt, np.uint32).astype(np.int64)
-        counters[name] = dict(rays=cnt, node_visits=int(v.sum()), tri_tests=int(t.sum()),
-                              bytes=cnt * (RAY_BYTES + HIT_BYTES) + int(v.sum()) * NODE_BYTES + int(t.sum()) * TRI_BYTES)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for k in range(args.steps):
+            step(evs[k])
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms_per_step = start.elapsed_time(stop) / args.steps
+        diff_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        shad_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        dom = "shadow" if shad_ms >= diff_ms else "diffuse"
+        dom_ms = shad_ms if dom == "shadow" else diff_ms
+        achieved = counters[dom]["bytes"] / (dom_ms * 1e-3) / 1e9
+        kernel = f"k_trace<watertight,{'any' if dom == 'shadow' else 'closest'},{form}>"
+        results[form] = {
+            "Mrays_s": live_rays / (ms_per_step * 1e-3) / 1e6,
+            "ms_per_step": ms_per_step, "diffuse_ms": diff_ms, "shadow_ms": shad_ms,
+            "diffuse_Mrays_s": n_live / (diff_ms * 1e-3) / 1e6, "shadow_Mrays_s": 4 * n_live / (shad_ms * 1e-3) / 1e6,
+            "counters": counters,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.traffic_file, kernel),
+                         "kernel": kernel, "launch_ms": round(dom_ms, 4),
+                         "algorithmic_bytes_per_launch": counters[dom]["bytes"],
+                         "bytes_per_ray": "32 ray + 16 hit + %d x node visits + 48 x triangle tests" % node_bytes},
+        }
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record(stream)
-        sc.trace(d_diff.data_ptr(), n_diff, d_hd.data_ptr(), 0, sp)
-        if ev is not None:
-            ev[1].record(stream)
-        sc.trace(d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY, sp)
-        if ev is not None:
-            ev[2].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for k in range(args.steps):
-        step(evs[k])
-    stop.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(stop)
-    diff_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    shad_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-    ms_per_step = total_ms / args.steps
-    mrays = live_rays / (ms_per_step * 1e-3) / 1e6
-
-    dom = "shadow" if shad_ms >= diff_ms else "diffuse"
-    dom_ms = shad_ms if dom == "shadow" else diff_ms
-    achieved = counters[dom]["bytes"] / (dom_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.traffic_file)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"k_trace<watertight,{'any' if dom == 'shadow' else 'closest'}>",
-                "launch_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": counters[dom]["bytes"]}
+    best = max(results, key=lambda f: results[f]["Mrays_s"])
+    res = results[best]
 
     # CPU baseline: the reference traversal (raytracer.glsl:246-295 restated in
     # C: 48-B pre-order nodes, RefMT leaf test, no pruning) on this host's cores
@@ -171,14 +190,17 @@ def main() -> int:
                          f"semantics (RefMT, no pruning), {cpu_s:.2f} s",
                "watertight_pruned_Mrays_s": round(live_rays / cpu2_s / 1e6, 3)}
 
+    def rounded(d):
+        return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
     out = {
         "metric": "Mrays/s (2M-tri terrain, diffuse+shadow) learned vs plain, hit IDs exact",
-        "value": round(mrays, 2),
+        "value": round(res["Mrays_s"], 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(res["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -186,15 +208,16 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "terrain 1024x1024 (2,093,058 tris), 1920x1080 primary hits -> "
                                "1 diffuse (closest) + 4 shadow (any) rays each",
-                   "form": "plain", "rays_per_step": live_rays, "max_leaf_size": 4, "search_radius": 16},
-        "roofline": roofline,
+                   "form": best, "rays_per_step": live_rays, "max_leaf_size": 4, "search_radius": 16},
+        "roofline": res["roofline"],
         "cpu_baseline": cpu,
-        "detail": {"diffuse_ms": round(diff_ms, 4), "shadow_ms": round(shad_ms, 4),
-                   "diffuse_Mrays_s": round(n_live / (diff_ms * 1e-3) / 1e6, 1),
-                   "shadow_Mrays_s": round(4 * n_live / (shad_ms * 1e-3) / 1e6, 1),
+        "detail": {"forms": {f: rounded({k: v for k, v in r.items() if k != "roofline"}) | {
+                        "roofline_frac": r["roofline"]["frac"]} for f, r in results.items()},
+                   "speedup_vs_plain": round(res["Mrays_s"] / results["plain"]["Mrays_s"], 3)
+                   if "plain" in results else None,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
-                   "depth": info.depth, "trav_nodes": info.n_trav_nodes,
-                   "counters": counters, "setup_s": round(setup_s, 1)},
+                   "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
+                   "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},
     }
     print(json.dumps(out))
     sc.release()

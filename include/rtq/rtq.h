@@ -84,7 +84,9 @@ typedef struct rtq_build_params {
     uint32_t max_leaf_size;   /* triangles per traversal leaf, 1..8 (the reference
                                  node array keeps one triangle per leaf regardless) */
     uint32_t flags;           /* RTQ_BUILD_* */
-    uint32_t reserved;
+    uint32_t node_width;      /* traversal layout: 2 = binary 64-B nodes (default, 0 means 2),
+                                 8 = compressed 8-wide 128-B nodes with 8-bit outward-rounded
+                                 child boxes; the binary layout is built in both cases */
 } rtq_build_params;
 
 void rtq_build_params_default(rtq_build_params* p);
@@ -104,6 +106,10 @@ typedef struct rtq_bvh_info {
     uint32_t max_leaf_size;
     float box_pad;               /* outward padding of traversal boxes (world units) */
     float build_ms;              /* device time of the last build */
+    uint32_t node_width;         /* layout traced by default: 2 or 8 */
+    uint32_t n_wide_nodes;       /* compressed 8-wide nodes (0 unless node_width == 8) */
+    uint32_t wide_depth;         /* levels of the 8-wide tree */
+    uint32_t reserved;
 } rtq_bvh_info;
 
 rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out);
@@ -150,6 +156,8 @@ rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t
                                        (raytracer.glsl:102-147: back-face culled,
                                        |det| < 1e-4 rejected, vertices 1/2 swapped);
                                        default leaf test is watertight FP32 */
+#define RTQ_TRACE_BINARY      0x4u  /* trace the binary layout even if the scene was
+                                       built with node_width 8 */
 
 /* Trace n_rays rays against the scene's BVH.  Replaces the traversal of
  * getClosestHitBVH (raytracer.glsl:246-295) dispatched by drawOneFrame

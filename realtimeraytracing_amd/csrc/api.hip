@@ -14,7 +14,6 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm);
 rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_codes);
 rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,
                  uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st);
-extern uint32_t g_refill_threshold;
 
 rtq_status fail(rtq_ctx* ctx, rtq_status s, const char* fmt, ...) {
     if (ctx) {
@@ -60,8 +59,11 @@ void free_build(Scene* s) {
     release(s->ref_nodes);
     release(s->nodes);
     release(s->tris);
+    release(s->wide);
     s->built = false;
     s->n_trav = 0;
+    s->n_wide = 0;
+    s->wide_depth = 0;
 }
 
 static void free_scene(Scene* s) {
@@ -269,7 +271,11 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
     }
     if (const char* e = getenv("RTQ_REFILL")) {
         const int v = atoi(e);
-        if (v >= 1 && v <= 64) g_refill_threshold = (uint32_t)v;
+        if (v >= 1 && v <= 64) ctx->refill = (uint32_t)v;
+    }
+    if (const char* e = getenv("RTQ_CHUNK")) {
+        const int v = atoi(e);
+        if (v >= 1 && v <= 4096) ctx->chunk = (uint32_t)v;
     }
     *out = ctx;
     return RTQ_OK;
@@ -357,7 +363,7 @@ void rtq_build_params_default(rtq_build_params* p) {
     p->search_radius = 16;
     p->max_leaf_size = 4;
     p->flags = 0;
-    p->reserved = 0;
+    p->node_width = 2;
 }
 
 rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params* params) {
@@ -370,6 +376,8 @@ rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params
     if (p.search_radius < 1 || p.search_radius > 32) return fail(ctx, RTQ_EINVAL, "bvh_build: search_radius must be 1..32");
     if (p.max_leaf_size < 1 || p.max_leaf_size > kMaxLeaf) return fail(ctx, RTQ_EINVAL, "bvh_build: max_leaf_size must be 1..8");
     if (p.flags & ~RTQ_BUILD_REF_PADDED_BOUNDS) return fail(ctx, RTQ_EINVAL, "bvh_build: unknown flags 0x%x", p.flags);
+    if (p.node_width == 0) p.node_width = 2;
+    if (p.node_width != 2 && p.node_width != 8) return fail(ctx, RTQ_EINVAL, "bvh_build: node_width must be 2 or 8");
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
     rtq_status st = build_scene(ctx, s, p);
     if (st != RTQ_OK) free_build(s);
@@ -389,6 +397,10 @@ rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out) 
     out->max_leaf_size = s->params.max_leaf_size;
     out->box_pad = s->box_pad;
     out->build_ms = s->build_ms;
+    out->node_width = s->params.node_width;
+    out->n_wide_nodes = s->n_wide;
+    out->wide_depth = s->wide_depth;
+    out->reserved = 0;
     return RTQ_OK;
 }
 
@@ -470,7 +482,8 @@ static rtq_status trace_common(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* r
     Scene* s = find_scene(ctx, scene_id);
     if (!s) return fail(ctx, RTQ_EINVAL, "trace: unknown scene %u", scene_id);
     if (!s->built) return fail(ctx, RTQ_ENOTBUILT, "trace: scene %u has no BVH", scene_id);
-    if (flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT)) return fail(ctx, RTQ_EINVAL, "trace: unknown flags 0x%x", flags);
+    if (flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT | RTQ_TRACE_BINARY))
+        return fail(ctx, RTQ_EINVAL, "trace: unknown flags 0x%x", flags);
     if (n == 0) return RTQ_OK;
     if (!rays || !hits) return fail(ctx, RTQ_EINVAL, "trace: null ray or hit buffer");
     RTQ_HIP(ctx, hipSetDevice(ctx->device));

@@ -621,6 +621,10 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
                            s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, s->nodes);
         RTQ_HIP(ctx, hipGetLastError());
     }
+    if (prm.node_width == 8) {
+        rs = build_wide(ctx, s, fl, st);
+        if (rs != RTQ_OK) return rs;
+    }
     RTQ_HIP(ctx, hipEventRecord(ev1, st));
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     float ms = 0.0f;

@@ -56,6 +56,9 @@ struct Scene {
     float4* nodes = nullptr;            // traversal nodes, 4 float4 (64 B) each
     float4* tris = nullptr;             // traversal triangles, leaf order, 3 float4 (48 B) each
     uint32_t n_trav = 0;
+    float4* wide = nullptr;             // compressed 8-wide nodes, 8 float4 (128 B) each
+    uint32_t n_wide = 0;
+    uint32_t wide_depth = 0;            // levels of the wide tree
     uint32_t depth = 0;
     uint32_t ploc_passes = 0;
     float box_pad = 0.0f;
@@ -75,6 +78,10 @@ struct rtq_ctx {
     uint32_t* counter = nullptr;
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
+    // traversal launch settings: refill a wave once `refill` lanes are idle,
+    // taking `chunk` consecutive rays per atomic on the global counter
+    uint32_t refill = 16;
+    uint32_t chunk = 128;
 };
 
 namespace rtq {
@@ -97,6 +104,8 @@ rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
 // Exclusive scan of pairs of u32 (component-wise).
 rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total,
                       hipStream_t st);
+// Compressed 8-wide nodes from the cluster arrays (fl: first leaf rank per cluster).
+rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
                             uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st);

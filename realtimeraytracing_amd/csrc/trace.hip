@@ -37,7 +37,8 @@ struct TraceArgs {
     uint32_t* counter;
     uint32_t* spill;
     uint32_t spill_stride;
-    uint32_t refill;
+    uint32_t refill;     // refill once this many lanes of a wave are idle
+    uint32_t chunk;      // rays taken from the global counter per atomic
     uint32_t* stat_nodes;
     uint32_t* stat_tris;
 };
@@ -122,7 +123,35 @@ __device__ __forceinline__ bool refmt_test(float4 q0, float4 q1, float4 q2, floa
     return true;
 }
 
-template <int LEAF, bool ANY, bool STATS>
+__device__ __forceinline__ uint4 bits4(float4 v) {
+    return make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w));
+}
+
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+// 19-comparator sorting network for 8 keys (optimal size, depth 6).
+__device__ __forceinline__ void sort8(uint32_t* k) {
+    cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
+    cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
+    cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
+    cswap(k[2], k[4]); cswap(k[3], k[5]);
+    cswap(k[1], k[4]); cswap(k[3], k[6]);
+    cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
+}
+
+// child ref of slot s (0..7) from the two ref quads, as a select tree
+__device__ __forceinline__ uint32_t slot_ref(uint4 a, uint4 b, uint32_t s) {
+    const uint4 q = (s & 4u) ? b : a;
+    const uint32_t lo = (s & 1u) ? q.y : q.x;
+    const uint32_t hi = (s & 1u) ? q.w : q.z;
+    return (s & 2u) ? hi : lo;
+}
+
+template <int LEAF, bool ANY, bool STATS, bool WIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     __shared__ uint32_t s_stack[kStackLds][kTraceThreads];
     const uint32_t tid = threadIdx.x;
@@ -131,7 +160,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     const size_t stride = a.spill_stride;
 
     int rid = -1;
-    bool exhausted = false;
+    // wave-uniform ray pool [pool_next, pool_end); dry once the counter passed n
+    uint32_t pool_next = 0, pool_end = 0;
+    bool dry = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     float ix = 0, iy = 0, iz = 0, oix = 0, oiy = 0, oiz = 0;
     float tmin = 0, tbest = 0;
@@ -179,19 +210,30 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     };
 
     for (;;) {
-        // ---- refill idle lanes from the global ray counter
-        const bool need = (rid < 0) && !exhausted;
+        // ---- refill idle lanes from the wave's pool of consecutive rays; the
+        // pool is topped up with one atomic on the global counter per chunk
+        const bool need = (rid < 0) && !dry;
         const uint64_t need_mask = __ballot(need);
         if (need_mask) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need_mask) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(a.counter, (uint32_t)__popcll(need_mask));
-            base = __shfl(base, (int)leader);
-            if (need) {
-                const uint32_t r = base + __builtin_amdgcn_mbcnt_hi(
-                                              (uint32_t)(need_mask >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)need_mask, 0u));
-                if (r < a.n) {
+            if (pool_next == pool_end) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need_mask) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(a.counter, a.chunk);
+                base = __shfl(base, (int)leader);
+                if (base >= a.n) {
+                    dry = true;
+                } else {
+                    pool_next = base;
+                    pool_end = min(base + a.chunk, a.n);
+                }
+            }
+            if (!dry) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need_mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)need_mask, 0u));
+                const uint32_t avail = pool_end - pool_next;
+                const uint32_t r = pool_next + rank;
+                pool_next += min((uint32_t)__popcll(need_mask), avail);
+                if (need && rank < avail) {
                     rid = (int)r;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays + r);
                     const float4 r0 = rp[0], r1 = rp[1];
@@ -231,17 +273,67 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                         if (dkz == 0.0f) cur = kEmpty;  // zero direction: nothing can be hit
                     }
                     if (!(tmin <= tbest)) cur = kEmpty;  // empty interval
-                } else {
-                    exhausted = true;
                 }
             }
         }
+        // no lane holds a ray only once the counter has run dry
         if (__ballot(rid >= 0) == 0) break;
 
         // ---- traversal steps until enough lanes are idle
         for (;;) {
             if (rid >= 0) {
                 // descend through internal nodes
+                if constexpr (WIDE) {
+                    while (cur != kEmpty && !(cur & kLeafBit)) {
+                        const float4* nd = a.nodes + 8ull * cur;
+                        const float4 h = nd[0];
+                        const uint4 qx = bits4(nd[1]), qy = bits4(nd[2]), qz = bits4(nd[3]);
+                        const uint4 c03 = bits4(nd[4]), c47 = bits4(nd[5]);
+                        if (STATS) ++n_nodes;
+                        const uint32_t meta = __float_as_uint(h.w);
+                        // child plane distance = q * (2^e * inv) + (origin - o) * inv
+                        const float scx = __uint_as_float((meta & 0xFFu) << 23) * ix;
+                        const float scy = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * iy;
+                        const float scz = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * iz;
+                        const float bx = __builtin_fmaf(h.x, ix, -oix);
+                        const float by = __builtin_fmaf(h.y, iy, -oiy);
+                        const float bz = __builtin_fmaf(h.z, iz, -oiz);
+                        const uint32_t valid = meta >> 24;
+                        uint32_t key[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int w = j >> 2, sh = 8 * (j & 3);
+                            const float lx = (float)(((w ? qx.y : qx.x) >> sh) & 0xFFu);
+                            const float hx = (float)(((w ? qx.w : qx.z) >> sh) & 0xFFu);
+                            const float ly = (float)(((w ? qy.y : qy.x) >> sh) & 0xFFu);
+                            const float hy = (float)(((w ? qy.w : qy.z) >> sh) & 0xFFu);
+                            const float lz = (float)(((w ? qz.y : qz.x) >> sh) & 0xFFu);
+                            const float hz = (float)(((w ? qz.w : qz.z) >> sh) & 0xFFu);
+                            const float t0x = __builtin_fmaf(lx, scx, bx), t1x = __builtin_fmaf(hx, scx, bx);
+                            const float t0y = __builtin_fmaf(ly, scy, by), t1y = __builtin_fmaf(hy, scy, by);
+                            const float t0z = __builtin_fmaf(lz, scz, bz), t1z = __builtin_fmaf(hz, scz, bz);
+                            const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+                            const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+                            const bool hit = (tn <= tf) && ((valid >> j) & 1u);
+                            // order key: entry distance (non-negative floats order as uints), slot in the low bits
+                            key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
+                        }
+                        sort8(key);
+                        const uint32_t nh = (key[0] != 0xFFFFFFFFu) + (key[1] != 0xFFFFFFFFu) +
+                                            (key[2] != 0xFFFFFFFFu) + (key[3] != 0xFFFFFFFFu) +
+                                            (key[4] != 0xFFFFFFFFu) + (key[5] != 0xFFFFFFFFu) +
+                                            (key[6] != 0xFFFFFFFFu) + (key[7] != 0xFFFFFFFFu);
+                        if (nh == 0) {
+                            cur = pop();
+                            continue;
+                        }
+                        // farthest first onto the stack, so the next pop is the second nearest
+#pragma unroll
+                        for (int j = 7; j >= 1; --j)
+                            if ((uint32_t)j < nh) push(slot_ref(c03, c47, key[j] & 7u));
+                        cur = slot_ref(c03, c47, key[0] & 7u);
+                    }
+                } else
                 while (cur != kEmpty && !(cur & kLeafBit)) {
                     const float4* nd = a.nodes + 4ull * cur;
                     const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
@@ -301,9 +393,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                 }
                 if (cur == kEmpty) finish();
             }
-            const uint64_t idle = __ballot(rid < 0 && !exhausted);
+            const uint64_t idle = __ballot(rid < 0);
             const uint64_t active = __ballot(rid >= 0);
-            if (active == 0 || (uint32_t)__popcll(idle) >= a.refill) break;
+            if (active == 0 || (!dry && (uint32_t)__popcll(idle) >= a.refill)) break;
         }
     }
     // A wave leaves the loop only after its last fetch ran past n, so when every
@@ -320,18 +412,19 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     }
 }
 
-template <int LEAF, bool ANY, bool STATS>
+template <int LEAF, bool ANY, bool STATS, bool WIDE>
 rtq_status launch(rtq_ctx* ctx, const Scene* s, const TraceArgs& base, uint32_t n, hipStream_t st) {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, ANY, STATS>,
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, ANY, STATS, WIDE>,
                                                                kTraceThreads, 0);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
     if (blocks == 0) blocks = 1;
     const uint64_t threads = blocks * kTraceThreads;
-    // stack entries needed: one per tree level below the root, plus one
-    const uint64_t depth_entries = (uint64_t)s->depth + 2;
+    // Stack bound: binary nodes push one entry per tree level below the root;
+    // a wide node pushes at most 7 per level.
+    const uint64_t depth_entries = WIDE ? 7ull * s->wide_depth + 2 : (uint64_t)s->depth + 2;
     const uint64_t spill_words = depth_entries > kStackLds ? (depth_entries - kStackLds) * threads : 1;
     if (spill_words > ctx->spill_words) {
         if (ctx->spill) {
@@ -344,40 +437,52 @@ rtq_status launch(rtq_ctx* ctx, const Scene* s, const TraceArgs& base, uint32_t 
         ctx->spill_words = spill_words;
     }
     TraceArgs args = base;
+    args.nodes = WIDE ? s->wide : s->nodes;
     args.counter = ctx->counter;
     args.spill = ctx->spill;
     args.spill_stride = (uint32_t)threads;
-    hipLaunchKernelGGL((k_trace<LEAF, ANY, STATS>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st, args);
+    hipLaunchKernelGGL((k_trace<LEAF, ANY, STATS, WIDE>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st, args);
     RTQ_HIP(ctx, hipGetLastError());
     return RTQ_OK;
 }
 
-}  // namespace
+template <int LEAF, bool ANY, bool STATS>
+rtq_status launch_layout(rtq_ctx* ctx, const Scene* s, const TraceArgs& a, uint32_t n, bool wide, hipStream_t st) {
+    return wide ? launch<LEAF, ANY, STATS, true>(ctx, s, a, n, st) : launch<LEAF, ANY, STATS, false>(ctx, s, a, n, st);
+}
 
-uint32_t g_refill_threshold = 16;
+}  // namespace
 
 rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,
                  uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st) {
     if (n == 0) return RTQ_OK;
-    if (n >= 0xFFFFFFFFull) return fail(ctx, RTQ_ECAPACITY, "trace: n_rays must be < 2^32 - 1");
+    // the counter runs past n by at most one chunk per wave: n < 2^31 keeps it in range
+    if (n > 0x80000000ull) return fail(ctx, RTQ_ECAPACITY, "trace: n_rays must be <= 2^31");
     TraceArgs args{};
-    args.nodes = s->nodes;
     args.tris = s->tris;
     args.rays = rays;
     args.hits = hits;
     args.n = (uint32_t)n;
-    args.refill = g_refill_threshold;
+    args.refill = ctx->refill;
+    args.chunk = ctx->chunk;
     args.stat_nodes = stat_nodes;
     args.stat_tris = stat_tris;
     const bool any = flags & RTQ_TRACE_ANY;
     const bool refmt = flags & RTQ_TRACE_LEAF_REFMT;
+    const bool wide = s->wide != nullptr && !(flags & RTQ_TRACE_BINARY);
     const uint32_t nn = (uint32_t)n;
     if (!stats) {
-        if (!refmt) return any ? launch<0, true, false>(ctx, s, args, nn, st) : launch<0, false, false>(ctx, s, args, nn, st);
-        return any ? launch<1, true, false>(ctx, s, args, nn, st) : launch<1, false, false>(ctx, s, args, nn, st);
+        if (!refmt)
+            return any ? launch_layout<0, true, false>(ctx, s, args, nn, wide, st)
+                       : launch_layout<0, false, false>(ctx, s, args, nn, wide, st);
+        return any ? launch_layout<1, true, false>(ctx, s, args, nn, wide, st)
+                   : launch_layout<1, false, false>(ctx, s, args, nn, wide, st);
     }
-    if (!refmt) return any ? launch<0, true, true>(ctx, s, args, nn, st) : launch<0, false, true>(ctx, s, args, nn, st);
-    return any ? launch<1, true, true>(ctx, s, args, nn, st) : launch<1, false, true>(ctx, s, args, nn, st);
+    if (!refmt)
+        return any ? launch_layout<0, true, true>(ctx, s, args, nn, wide, st)
+                   : launch_layout<0, false, true>(ctx, s, args, nn, wide, st);
+    return any ? launch_layout<1, true, true>(ctx, s, args, nn, wide, st)
+               : launch_layout<1, false, true>(ctx, s, args, nn, wide, st);
 }
 
 }  // namespace rtq

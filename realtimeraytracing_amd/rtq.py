@@ -37,6 +37,7 @@ RTQ_BUILD_REF_PADDED_BOUNDS = 0x1
 RTQ_TRACE_CLOSEST = 0x0
 RTQ_TRACE_ANY = 0x1
 RTQ_TRACE_LEAF_REFMT = 0x2
+RTQ_TRACE_BINARY = 0x4
 
 EXPORTED = [
     "rtq_abi_version", "rtq_status_string", "rtq_create", "rtq_destroy", "rtq_last_error",
@@ -49,14 +50,16 @@ EXPORTED = [
 
 class BuildParams(ctypes.Structure):
     _fields_ = [("search_radius", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
-                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("node_width", ctypes.c_uint32)]
 
 
 class BvhInfo(ctypes.Structure):
     _fields_ = [("n_tris", ctypes.c_uint32), ("n_ref_nodes", ctypes.c_uint32),
                 ("n_trav_nodes", ctypes.c_uint32), ("depth", ctypes.c_uint32),
                 ("ploc_iterations", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
-                ("box_pad", ctypes.c_float), ("build_ms", ctypes.c_float)]
+                ("box_pad", ctypes.c_float), ("build_ms", ctypes.c_float),
+                ("node_width", ctypes.c_uint32), ("n_wide_nodes", ctypes.c_uint32),
+                ("wide_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Pinhole(ctypes.Structure):
@@ -177,8 +180,9 @@ class Scene:
     def release(self) -> None:
         self.ctx.check(self.ctx.lib.rtq_scene_release(self.ctx.h, self.id))
 
-    def build(self, search_radius: int = 16, max_leaf_size: int = 4, flags: int = 0) -> "BvhInfo":
-        p = BuildParams(search_radius, max_leaf_size, flags, 0)
+    def build(self, search_radius: int = 16, max_leaf_size: int = 4, flags: int = 0,
+              node_width: int = 2) -> "BvhInfo":
+        p = BuildParams(search_radius, max_leaf_size, flags, node_width)
         self.ctx.check(self.ctx.lib.rtq_bvh_build(self.ctx.h, self.id, ctypes.byref(p)))
         return self.info()
 

@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--forms", default="plain,compressed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
+                         "known traffic to scale rocprof FETCH_SIZE / WRITE_SIZE readings")
     return ap.parse_args()
 
 
@@ -222,6 +225,15 @@ def main() -> int:
     print(json.dumps(out))
     sc.release()
     ctx.close()
+    if args.copy_calibration_mb > 0:
+        nbytes = args.copy_calibration_mb << 20
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes}))
     return 0
 
 

@@ -151,11 +151,15 @@ __device__ __forceinline__ uint32_t slot_ref(uint4 a, uint4 b, uint32_t s) {
     return (s & 2u) ? hi : lo;
 }
 
+__device__ __forceinline__ bool is_leaf_ref(uint32_t r) { return (r & kLeafBit) && r != kEmpty; }
+__device__ __forceinline__ bool is_node_ref(uint32_t r) { return !(r & kLeafBit); }
+
 template <int LEAF, bool ANY, bool STATS, bool WIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
-    __shared__ uint32_t s_stack[kStackLds][kTraceThreads];
+    __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
+    uint32_t* const lds = s_stack + tid;  // entry k at lds[k * kTraceThreads]: lanes on distinct banks
     uint32_t* const spill = a.spill + (size_t)blockIdx.x * kTraceThreads + tid;
     const size_t stride = a.spill_stride;
 
@@ -170,18 +174,26 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     float sx = 0, sy = 0, sz = 0, opx = 0, opy = 0, opz = 0;
     int best_id = -1;
     float best_t = 0, best_a = 0, best_b = 0, best_c = 0;
-    uint32_t cur = kEmpty, sp = 0;
+    // cur: next node to visit (or a leaf waiting behind the postponed one);
+    // leaf: postponed leaf; kEmpty in both once the stack ran out
+    uint32_t cur = kEmpty, leaf = kEmpty, sp = 0;
     uint32_t n_nodes = 0, n_tests = 0;
 
     auto push = [&](uint32_t v) {
-        if (sp < kStackLds) s_stack[sp][tid] = v;
-        else spill[(size_t)(sp - kStackLds) * stride] = v;
+        if (sp < kStackLds) {
+            lds[sp * kTraceThreads] = v;
+        } else {
+            __hip_atomic_store(spill + (size_t)(sp - kStackLds) * stride, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
         ++sp;
     };
     auto pop = [&]() -> uint32_t {
         if (sp == 0) return kEmpty;
         --sp;
-        return sp < kStackLds ? s_stack[sp][tid] : spill[(size_t)(sp - kStackLds) * stride];
+        if (sp < kStackLds) return lds[sp * kTraceThreads];
+        return __hip_atomic_load(spill + (size_t)(sp - kStackLds) * stride, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     auto finish = [&]() {
         rtq_hit h;
@@ -207,6 +219,94 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
             if (a.stat_tris) a.stat_tris[rid] = n_tests;
         }
         rid = -1;
+    };
+    // One internal-node visit: cur <- nearest hit child (farther ones pushed) or a pop.
+    auto visit = [&]() {
+        if constexpr (WIDE) {
+            const float4* nd = a.nodes + 8ull * cur;
+            const float4 h = nd[0];
+            const uint4 qx = bits4(nd[1]), qy = bits4(nd[2]), qz = bits4(nd[3]);
+            const uint4 c03 = bits4(nd[4]), c47 = bits4(nd[5]);
+            if (STATS) ++n_nodes;
+            const uint32_t meta = __float_as_uint(h.w);
+            // child plane distance = q * (2^e * inv) + (origin - o) * inv
+            const float scx = __uint_as_float((meta & 0xFFu) << 23) * ix;
+            const float scy = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * iy;
+            const float scz = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * iz;
+            const float bx = __builtin_fmaf(h.x, ix, -oix);
+            const float by = __builtin_fmaf(h.y, iy, -oiy);
+            const float bz = __builtin_fmaf(h.z, iz, -oiz);
+            const uint32_t valid = meta >> 24;
+            uint32_t key[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int w = j >> 2, sh = 8 * (j & 3);
+                const float lx = (float)(((w ? qx.y : qx.x) >> sh) & 0xFFu);
+                const float hx = (float)(((w ? qx.w : qx.z) >> sh) & 0xFFu);
+                const float ly = (float)(((w ? qy.y : qy.x) >> sh) & 0xFFu);
+                const float hy = (float)(((w ? qy.w : qy.z) >> sh) & 0xFFu);
+                const float lz = (float)(((w ? qz.y : qz.x) >> sh) & 0xFFu);
+                const float hz = (float)(((w ? qz.w : qz.z) >> sh) & 0xFFu);
+                const float t0x = __builtin_fmaf(lx, scx, bx), t1x = __builtin_fmaf(hx, scx, bx);
+                const float t0y = __builtin_fmaf(ly, scy, by), t1y = __builtin_fmaf(hy, scy, by);
+                const float t0z = __builtin_fmaf(lz, scz, bz), t1z = __builtin_fmaf(hz, scz, bz);
+                const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+                const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+                const bool hit = (tn <= tf) && ((valid >> j) & 1u);
+                // order key: entry distance (non-negative floats order as uints), slot in the low bits
+                key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
+            }
+            if constexpr (ANY) {
+                // any hit: nearest child first, the others in slot order
+                uint32_t m = min(min(min(key[0], key[1]), min(key[2], key[3])),
+                                 min(min(key[4], key[5]), min(key[6], key[7])));
+                if (m == 0xFFFFFFFFu) {
+                    cur = pop();
+                    return;
+                }
+                const uint32_t first = m & 7u;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (key[j] != 0xFFFFFFFFu && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                cur = slot_ref(c03, c47, first);
+            } else {
+                sort8(key);
+                if (key[0] == 0xFFFFFFFFu) {
+                    cur = pop();
+                    return;
+                }
+                // farthest first onto the stack, so the next pop is the second nearest
+#pragma unroll
+                for (int j = 7; j >= 1; --j)
+                    if (key[j] != 0xFFFFFFFFu) push(slot_ref(c03, c47, key[j] & 7u));
+                cur = slot_ref(c03, c47, key[0] & 7u);
+            }
+        } else {
+            const float4* nd = a.nodes + 4ull * cur;
+            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+            if (STATS) ++n_nodes;
+            const float ax0 = __builtin_fmaf(n0.x, ix, -oix), bx0 = __builtin_fmaf(n0.y, ix, -oix);
+            const float ay0 = __builtin_fmaf(n0.z, iy, -oiy), by0 = __builtin_fmaf(n0.w, iy, -oiy);
+            const float az0 = __builtin_fmaf(n1.x, iz, -oiz), bz0 = __builtin_fmaf(n1.y, iz, -oiz);
+            const float ax1 = __builtin_fmaf(n1.z, ix, -oix), bx1 = __builtin_fmaf(n1.w, ix, -oix);
+            const float ay1 = __builtin_fmaf(n2.x, iy, -oiy), by1 = __builtin_fmaf(n2.y, iy, -oiy);
+            const float az1 = __builtin_fmaf(n2.z, iz, -oiz), bz1 = __builtin_fmaf(n2.w, iz, -oiz);
+            const float tn0 = fmaxf(fmaxf(fminf(ax0, bx0), fminf(ay0, by0)), fmaxf(fminf(az0, bz0), tmin));
+            const float tf0 = fminf(fminf(fmaxf(ax0, bx0), fmaxf(ay0, by0)), fminf(fmaxf(az0, bz0), tbest));
+            const float tn1 = fmaxf(fmaxf(fminf(ax1, bx1), fminf(ay1, by1)), fmaxf(fminf(az1, bz1), tmin));
+            const float tf1 = fminf(fminf(fmaxf(ax1, bx1), fmaxf(ay1, by1)), fminf(fmaxf(az1, bz1), tbest));
+            const bool h0 = tn0 <= tf0, h1 = tn1 <= tf1;
+            const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
+            if (h0 && h1) {
+                const bool first0 = tn0 <= tn1;
+                push(first0 ? r1 : r0);
+                cur = first0 ? r0 : r1;
+            } else if (h0 || h1) {
+                cur = h0 ? r0 : r1;
+            } else {
+                cur = pop();
+            }
+        }
     };
 
     for (;;) {
@@ -244,6 +344,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                     n_tests = 0;
                     sp = 0;
                     cur = 0;
+                    leaf = kEmpty;
                     // box-test reciprocals (zero components replaced by +-1e-30)
                     const float ex = fabsf(dx) < 1e-30f ? copysignf(1e-30f, dx) : dx;
                     const float ey = fabsf(dy) < 1e-30f ? copysignf(1e-30f, dy) : dy;
@@ -282,94 +383,26 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
         // ---- traversal steps until enough lanes are idle
         for (;;) {
             if (rid >= 0) {
-                // descend through internal nodes
-                if constexpr (WIDE) {
-                    while (cur != kEmpty && !(cur & kLeafBit)) {
-                        const float4* nd = a.nodes + 8ull * cur;
-                        const float4 h = nd[0];
-                        const uint4 qx = bits4(nd[1]), qy = bits4(nd[2]), qz = bits4(nd[3]);
-                        const uint4 c03 = bits4(nd[4]), c47 = bits4(nd[5]);
-                        if (STATS) ++n_nodes;
-                        const uint32_t meta = __float_as_uint(h.w);
-                        // child plane distance = q * (2^e * inv) + (origin - o) * inv
-                        const float scx = __uint_as_float((meta & 0xFFu) << 23) * ix;
-                        const float scy = __uint_as_float(((meta >> 8) & 0xFFu) << 23) * iy;
-                        const float scz = __uint_as_float(((meta >> 16) & 0xFFu) << 23) * iz;
-                        const float bx = __builtin_fmaf(h.x, ix, -oix);
-                        const float by = __builtin_fmaf(h.y, iy, -oiy);
-                        const float bz = __builtin_fmaf(h.z, iz, -oiz);
-                        const uint32_t valid = meta >> 24;
-                        uint32_t key[8];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const int w = j >> 2, sh = 8 * (j & 3);
-                            const float lx = (float)(((w ? qx.y : qx.x) >> sh) & 0xFFu);
-                            const float hx = (float)(((w ? qx.w : qx.z) >> sh) & 0xFFu);
-                            const float ly = (float)(((w ? qy.y : qy.x) >> sh) & 0xFFu);
-                            const float hy = (float)(((w ? qy.w : qy.z) >> sh) & 0xFFu);
-                            const float lz = (float)(((w ? qz.y : qz.x) >> sh) & 0xFFu);
-                            const float hz = (float)(((w ? qz.w : qz.z) >> sh) & 0xFFu);
-                            const float t0x = __builtin_fmaf(lx, scx, bx), t1x = __builtin_fmaf(hx, scx, bx);
-                            const float t0y = __builtin_fmaf(ly, scy, by), t1y = __builtin_fmaf(hy, scy, by);
-                            const float t0z = __builtin_fmaf(lz, scz, bz), t1z = __builtin_fmaf(hz, scz, bz);
-                            const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-                            const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
-                            const bool hit = (tn <= tf) && ((valid >> j) & 1u);
-                            // order key: entry distance (non-negative floats order as uints), slot in the low bits
-                            key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
-                        }
-                        sort8(key);
-                        const uint32_t nh = (key[0] != 0xFFFFFFFFu) + (key[1] != 0xFFFFFFFFu) +
-                                            (key[2] != 0xFFFFFFFFu) + (key[3] != 0xFFFFFFFFu) +
-                                            (key[4] != 0xFFFFFFFFu) + (key[5] != 0xFFFFFFFFu) +
-                                            (key[6] != 0xFFFFFFFFu) + (key[7] != 0xFFFFFFFFu);
-                        if (nh == 0) {
-                            cur = pop();
-                            continue;
-                        }
-                        // farthest first onto the stack, so the next pop is the second nearest
-#pragma unroll
-                        for (int j = 7; j >= 1; --j)
-                            if ((uint32_t)j < nh) push(slot_ref(c03, c47, key[j] & 7u));
-                        cur = slot_ref(c03, c47, key[0] & 7u);
-                    }
-                } else
-                while (cur != kEmpty && !(cur & kLeafBit)) {
-                    const float4* nd = a.nodes + 4ull * cur;
-                    const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-                    if (STATS) ++n_nodes;
-                    const float ax0 = __builtin_fmaf(n0.x, ix, -oix), bx0 = __builtin_fmaf(n0.y, ix, -oix);
-                    const float ay0 = __builtin_fmaf(n0.z, iy, -oiy), by0 = __builtin_fmaf(n0.w, iy, -oiy);
-                    const float az0 = __builtin_fmaf(n1.x, iz, -oiz), bz0 = __builtin_fmaf(n1.y, iz, -oiz);
-                    const float ax1 = __builtin_fmaf(n1.z, ix, -oix), bx1 = __builtin_fmaf(n1.w, ix, -oix);
-                    const float ay1 = __builtin_fmaf(n2.x, iy, -oiy), by1 = __builtin_fmaf(n2.y, iy, -oiy);
-                    const float az1 = __builtin_fmaf(n2.z, iz, -oiz), bz1 = __builtin_fmaf(n2.w, iz, -oiz);
-                    const float tn0 = fmaxf(fmaxf(fminf(ax0, bx0), fminf(ay0, by0)), fmaxf(fminf(az0, bz0), tmin));
-                    const float tf0 = fminf(fminf(fmaxf(ax0, bx0), fmaxf(ay0, by0)), fminf(fmaxf(az0, bz0), tbest));
-                    const float tn1 = fmaxf(fmaxf(fminf(ax1, bx1), fminf(ay1, by1)), fmaxf(fminf(az1, bz1), tmin));
-                    const float tf1 = fminf(fminf(fmaxf(ax1, bx1), fmaxf(ay1, by1)), fminf(fmaxf(az1, bz1), tbest));
-                    const bool h0 = tn0 <= tf0, h1 = tn1 <= tf1;
-                    const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
-                    if (h0 && h1) {
-                        const bool first0 = tn0 <= tn1;
-                        push(first0 ? r1 : r0);
-                        cur = first0 ? r0 : r1;
-                    } else if (h0) {
-                        cur = r0;
-                    } else if (h1) {
-                        cur = r1;
-                    } else {
+                // Internal nodes until every lane of the wave holds a leaf: a lane
+                // that finds one postpones it and keeps descending (speculative
+                // traversal, Aila & Laine, HPG 2009).
+                bool searching = leaf == kEmpty;
+                while (is_node_ref(cur)) {
+                    visit();
+                    if (is_leaf_ref(cur) && leaf == kEmpty) {
+                        leaf = cur;
                         cur = pop();
+                        searching = false;
                     }
+                    if (!__ballot(searching)) break;
                 }
-                // one leaf
-                if (cur != kEmpty) {
-                    const uint32_t first = (cur >> 3) & 0x0FFFFFFFu;
-                    const uint32_t count = (cur & 7u) + 1u;
+                // postponed leaf, then any leaf that is waiting in cur
+                while (is_leaf_ref(leaf)) {
+                    const uint32_t first = (leaf >> 3) & 0x0FFFFFFFu;
+                    const uint32_t count = (leaf & 7u) + 1u;
                     bool done = false;
                     for (uint32_t k = 0; k < count; ++k) {
-                        const uint32_t ti = first + k;
-                        const float4* tp = a.tris + 3ull * ti;
+                        const float4* tp = a.tris + 3ull * (first + k);
                         const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
                         if (STATS) ++n_tests;
                         const int id = (int)__float_as_uint(p0.w);
@@ -389,9 +422,19 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                             if (ANY) { done = true; break; }
                         }
                     }
-                    cur = done ? kEmpty : pop();
+                    if (done) {
+                        cur = kEmpty;
+                        leaf = kEmpty;
+                        break;
+                    }
+                    if (is_leaf_ref(cur)) {
+                        leaf = cur;
+                        cur = pop();
+                    } else {
+                        leaf = kEmpty;
+                    }
                 }
-                if (cur == kEmpty) finish();
+                if (cur == kEmpty && leaf == kEmpty) finish();
             }
             const uint64_t idle = __ballot(rid < 0);
             const uint64_t active = __ballot(rid >= 0);

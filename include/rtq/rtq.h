@@ -177,6 +177,68 @@ rtq_status rtq_trace_stats(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_
 rtq_status rtq_trace_host(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays, uint64_t n_rays,
                           uint32_t flags, rtq_hit* hits);
 
+/* ---- batched traversal (no counterpart in the reference: one GL dispatch
+ * per frame and scene, application.cpp:220-246) ------------------------- */
+
+/* One ray set of a batch: its scene, its device buffers and its mode
+ * (RTQ_TRACE_ANY or closest hit).  The leaf test (RTQ_TRACE_LEAF_REFMT) and
+ * the node layout traced must be the same for every item of a launch. */
+typedef struct rtq_batch_item {
+    uint32_t scene_id;
+    uint32_t flags;
+    const rtq_ray* rays_dev;
+    rtq_hit* hits_dev;
+    uint64_t n_rays;
+} rtq_batch_item;
+
+#define RTQ_MAX_BATCH_ITEMS 64
+
+/* Up to RTQ_MAX_BATCH_ITEMS ray sets of uneven sizes over any scenes in ONE
+ * persistent launch: waves take chunks of consecutive rays from a single work
+ * counter over all sets, so a long set never leaves the chip idle behind a
+ * short one.  Asynchronous on `stream`. */
+rtq_status rtq_trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, void* stream);
+
+/* The same launch captured once in a hipGraph and replayed: create sizes the
+ * batch's own scratch (so replays allocate nothing and share nothing with
+ * other calls), captures, and instantiates; launch replays it on `stream`.
+ * Item buffers must stay valid for the batch's lifetime.  Replays of one batch
+ * on different streams must not overlap (they share the batch's scratch). */
+typedef struct rtq_batch rtq_batch;
+rtq_status rtq_batch_create(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, rtq_batch** out);
+rtq_status rtq_batch_launch(rtq_batch* batch, void* stream);
+void rtq_batch_destroy(rtq_batch* batch);
+
+/* ---- launch settings and autotuner ---------------------------------------- */
+
+typedef struct rtq_launch_params {
+    uint32_t refill;       /* refill a wave's idle lanes once this many (1..64) are idle */
+    uint32_t chunk;        /* consecutive rays a wave takes per atomic (1..4096) */
+    uint32_t prefer_wide;  /* trace 8-wide nodes when a scene has them */
+    uint32_t reserved;
+} rtq_launch_params;
+
+rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out);
+rtq_status rtq_launch_params_set(rtq_ctx* ctx, const rtq_launch_params* p);
+
+typedef struct rtq_tune_result {
+    rtq_launch_params best;
+    float best_ms;         /* batch time with the chosen settings */
+    float default_ms;      /* batch time with the settings in force before tuning */
+    uint32_t n_tried;      /* candidates timed (0: taken from the tuning cache) */
+    uint32_t from_cache;
+} rtq_tune_result;
+
+/* Times candidate launch settings (refill, chunk, node layout) on the given
+ * batch, `reps` launches each, applies the fastest to the context and records
+ * it under a key of (device, scene sizes, ray counts, modes); a later call
+ * with the same key reuses it without timing.  Writes the items' hit buffers. */
+rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, uint32_t reps,
+                        rtq_tune_result* out);
+/* Versioned, checksummed file of the tuning cache. */
+rtq_status rtq_tune_save(rtq_ctx* ctx, const char* path);
+rtq_status rtq_tune_load(rtq_ctx* ctx, const char* path);
+
 /* ---- ray generation (caller side of the path) --------------------------- */
 
 /* Pinhole camera given by precomputed basis vectors: for pixel (i, j) of a

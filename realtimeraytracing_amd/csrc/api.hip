@@ -260,11 +260,9 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
     ctx->device = hip_device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
-    // counter[0]: next ray of a persistent trace launch, counter[1]: blocks done;
-    // zero here, then reset by the last block of every launch
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&ctx->counter), 256) != hipSuccess ||
-        hipMemset(ctx->counter, 0, 256) != hipSuccess) {
+        scratch_alloc(ctx, &ctx->scratch) != RTQ_OK) {
+        scratch_free(&ctx->scratch);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return RTQ_EHIP;
@@ -287,8 +285,7 @@ void rtq_destroy(rtq_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->scenes) free_scene(kv.second);
     ctx->scenes.clear();
-    if (ctx->counter) (void)hipFree(ctx->counter);
-    if (ctx->spill) (void)hipFree(ctx->spill);
+    scratch_free(&ctx->scratch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -67,6 +67,17 @@ struct Scene {
 
 }  // namespace rtq
 
+namespace rtq {
+// Device scratch of persistent trace launches: counter[0] = next ray,
+// counter[1] = blocks done (both zero between launches: the last block of a
+// launch resets them), spill = the deep part of the traversal stacks.
+struct TraceScratch {
+    uint32_t* counter = nullptr;
+    uint32_t* spill = nullptr;
+    size_t spill_words = 0;
+};
+}  // namespace rtq
+
 struct rtq_ctx {
     int device = 0;
     int n_cu = 0;
@@ -74,14 +85,24 @@ struct rtq_ctx {
     std::string err;
     std::map<uint32_t, rtq::Scene*> scenes;
     uint32_t next_scene = 1;
-    // trace scratch: per-launch ray counter and the deep part of traversal stacks
-    uint32_t* counter = nullptr;
-    uint32_t* spill = nullptr;
-    size_t spill_words = 0;
+    rtq::TraceScratch scratch;
     // traversal launch settings: refill a wave once `refill` lanes are idle,
-    // taking `chunk` consecutive rays per atomic on the global counter
+    // taking `chunk` consecutive rays per atomic on the global counter; trace
+    // 8-wide nodes when a scene has them and prefer_wide is set
     uint32_t refill = 16;
     uint32_t chunk = 128;
+    bool prefer_wide = true;
+    // autotuner results: key -> launch settings
+    std::map<uint64_t, rtq_launch_params> tuned;
+};
+
+struct rtq_batch {
+    rtq_ctx* ctx = nullptr;
+    std::vector<rtq_batch_item> items;
+    rtq::TraceScratch scratch;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace rtq {
@@ -104,6 +125,13 @@ rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
 // Exclusive scan of pairs of u32 (component-wise).
 rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total,
                       hipStream_t st);
+// Trace entry points (trace.hip).  `scratch` null = the context's own.
+rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,
+                 uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st);
+rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
+                       bool size_only, hipStream_t st);
+rtq_status scratch_alloc(rtq_ctx* ctx, TraceScratch* sc);
+void scratch_free(TraceScratch* sc);
 // Compressed 8-wide nodes from the cluster arrays (fl: first leaf rank per cluster).
 rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.

@@ -28,11 +28,38 @@ namespace {
 constexpr int kTraceThreads = 256;
 constexpr uint32_t kStackLds = 16;
 
+// One ray set of a batched launch: rays [begin, begin + n) of the launch's
+// global index space trace against one scene.
+struct BatchItem {
+    const float4* nodes;
+    const float4* tris;
+    const rtq_ray* rays;
+    rtq_hit* hits;
+    uint32_t begin;
+    uint32_t n;
+    uint32_t any;
+    uint32_t pad;
+};
+static_assert(sizeof(BatchItem) == 48, "batch descriptor is 48 B");
+constexpr uint32_t kMaxBatchItems = 64;
+
+// Item descriptors of a batched launch, passed by value (3 KB of kernel
+// arguments) so a captured graph keeps them; empty for single-scene launches.
+template <bool MULTI>
+struct ItemTable {
+    BatchItem item[kMaxBatchItems];
+};
+template <>
+struct ItemTable<false> {
+    uint32_t unused;
+};
+
 struct TraceArgs {
     const float4* __restrict__ nodes;
     const float4* __restrict__ tris;
     const rtq_ray* __restrict__ rays;
     rtq_hit* __restrict__ hits;
+    uint32_t n_items;        // batched launches: items in the ItemTable argument
     uint32_t n;
     uint32_t* counter;
     uint32_t* spill;
@@ -154,14 +181,27 @@ __device__ __forceinline__ uint32_t slot_ref(uint4 a, uint4 b, uint32_t s) {
 __device__ __forceinline__ bool is_leaf_ref(uint32_t r) { return (r & kLeafBit) && r != kEmpty; }
 __device__ __forceinline__ bool is_node_ref(uint32_t r) { return !(r & kLeafBit); }
 
-template <int LEAF, bool ANY, bool STATS, bool WIDE>
-__global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
+// MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: rays come from
+// several items (scenes), each lane takes its scene and buffers from its item.
+template <int LEAF, int MODE, bool STATS, bool WIDE, bool MULTI>
+__global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<MULTI> table) {
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
+    __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     uint32_t* const lds = s_stack + tid;  // entry k at lds[k * kTraceThreads]: lanes on distinct banks
     uint32_t* const spill = a.spill + (size_t)blockIdx.x * kTraceThreads + tid;
     const size_t stride = a.spill_stride;
+    if constexpr (MULTI) {
+        for (uint32_t i = tid; i < a.n_items; i += kTraceThreads) s_items[i] = table.item[i];
+        __syncthreads();
+    }
+    // the lane's scene and ray set (kernel arguments unless batched)
+    const float4* nodes_p = a.nodes;
+    const float4* tris_p = a.tris;
+    rtq_hit* hits_p = a.hits;
+    uint32_t local = 0;              // ray index within its ray set
+    bool lane_any = MODE == 1;
 
     int rid = -1;
     // wave-uniform ray pool [pool_next, pool_end); dry once the counter passed n
@@ -213,7 +253,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
             h.u = 0.0f;
             h.v = 0.0f;
         }
-        a.hits[rid] = h;
+        hits_p[local] = h;
         if (STATS) {
             if (a.stat_nodes) a.stat_nodes[rid] = n_nodes;
             if (a.stat_tris) a.stat_tris[rid] = n_tests;
@@ -223,7 +263,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     // One internal-node visit: cur <- nearest hit child (farther ones pushed) or a pop.
     auto visit = [&]() {
         if constexpr (WIDE) {
-            const float4* nd = a.nodes + 8ull * cur;
+            const float4* nd = nodes_p + 8ull * cur;
             const float4 h = nd[0];
             const uint4 qx = bits4(nd[1]), qy = bits4(nd[2]), qz = bits4(nd[3]);
             const uint4 c03 = bits4(nd[4]), c47 = bits4(nd[5]);
@@ -256,7 +296,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                 // order key: entry distance (non-negative floats order as uints), slot in the low bits
                 key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
             }
-            if constexpr (ANY) {
+            if (MODE == 1 || (MODE == 2 && lane_any)) {
                 // any hit: nearest child first, the others in slot order
                 uint32_t m = min(min(min(key[0], key[1]), min(key[2], key[3])),
                                  min(min(key[4], key[5]), min(key[6], key[7])));
@@ -282,7 +322,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                 cur = slot_ref(c03, c47, key[0] & 7u);
             }
         } else {
-            const float4* nd = a.nodes + 4ull * cur;
+            const float4* nd = nodes_p + 4ull * cur;
             const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
             if (STATS) ++n_nodes;
             const float ax0 = __builtin_fmaf(n0.x, ix, -oix), bx0 = __builtin_fmaf(n0.y, ix, -oix);
@@ -335,7 +375,24 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                 pool_next += min((uint32_t)__popcll(need_mask), avail);
                 if (need && rank < avail) {
                     rid = (int)r;
-                    const float4* rp = reinterpret_cast<const float4*>(a.rays + r);
+                    const rtq_ray* rays_p = a.rays;
+                    local = r;
+                    if constexpr (MULTI) {
+                        // the item holding global ray r: last one with begin <= r
+                        uint32_t lo = 0, hi = a.n_items - 1;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi + 1) >> 1;
+                            if (s_items[mid].begin <= r) lo = mid; else hi = mid - 1;
+                        }
+                        const BatchItem& it = s_items[lo];
+                        nodes_p = it.nodes;
+                        tris_p = it.tris;
+                        rays_p = it.rays;
+                        hits_p = it.hits;
+                        local = r - it.begin;
+                        lane_any = it.any != 0;
+                    }
+                    const float4* rp = reinterpret_cast<const float4*>(rays_p + local);
                     const float4 r0 = rp[0], r1 = rp[1];
                     ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
                     dx = r1.x; dy = r1.y; dz = r1.z; tbest = r1.w;
@@ -402,7 +459,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                     const uint32_t count = (leaf & 7u) + 1u;
                     bool done = false;
                     for (uint32_t k = 0; k < count; ++k) {
-                        const float4* tp = a.tris + 3ull * (first + k);
+                        const float4* tp = tris_p + 3ull * (first + k);
                         const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
                         if (STATS) ++n_tests;
                         const int id = (int)__float_as_uint(p0.w);
@@ -419,7 +476,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
                             best_b = ub;
                             best_c = uc;
                             tbest = t;
-                            if (ANY) { done = true; break; }
+                            if (MODE == 1 || (MODE == 2 && lane_any)) { done = true; break; }
                         }
                     }
                     if (done) {
@@ -455,77 +512,156 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a) {
     }
 }
 
-template <int LEAF, bool ANY, bool STATS, bool WIDE>
-rtq_status launch(rtq_ctx* ctx, const Scene* s, const TraceArgs& base, uint32_t n, hipStream_t st) {
+// Persistent grid sized from occupancy, spill column sized from the stack
+// bound `depth_entries`, one launch (or only the scratch sizing).
+template <int LEAF, int MODE, bool STATS, bool WIDE, bool MULTI>
+rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const TraceArgs& base,
+                  const ItemTable<MULTI>& table, uint32_t n, bool size_only, hipStream_t st) {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, ANY, STATS, WIDE>,
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, MODE, STATS, WIDE, MULTI>,
                                                                kTraceThreads, 0);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
     if (blocks == 0) blocks = 1;
     const uint64_t threads = blocks * kTraceThreads;
-    // Stack bound: binary nodes push one entry per tree level below the root;
-    // a wide node pushes at most 7 per level.
-    const uint64_t depth_entries = WIDE ? 7ull * s->wide_depth + 2 : (uint64_t)s->depth + 2;
     const uint64_t spill_words = depth_entries > kStackLds ? (depth_entries - kStackLds) * threads : 1;
-    if (spill_words > ctx->spill_words) {
-        if (ctx->spill) {
-            RTQ_HIP(ctx, hipStreamSynchronize(st));
-            (void)hipFree(ctx->spill);
-            ctx->spill = nullptr;
-            ctx->spill_words = 0;
+    if (spill_words > sc.spill_words) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        RTQ_HIP(ctx, hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(ctx, RTQ_EINVAL, "trace: traversal scratch must be sized before graph capture");
+        if (sc.spill) {
+            // earlier launches on any stream may still use it
+            RTQ_HIP(ctx, hipDeviceSynchronize());
+            (void)hipFree(sc.spill);
+            sc.spill = nullptr;
+            sc.spill_words = 0;
         }
-        RTQ_HIP(ctx, dev_alloc(&ctx->spill, spill_words));
-        ctx->spill_words = spill_words;
+        RTQ_HIP(ctx, dev_alloc(&sc.spill, spill_words));
+        sc.spill_words = spill_words;
     }
+    if (size_only) return RTQ_OK;
     TraceArgs args = base;
-    args.nodes = WIDE ? s->wide : s->nodes;
-    args.counter = ctx->counter;
-    args.spill = ctx->spill;
+    args.counter = sc.counter;
+    args.spill = sc.spill;
     args.spill_stride = (uint32_t)threads;
-    hipLaunchKernelGGL((k_trace<LEAF, ANY, STATS, WIDE>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st, args);
+    args.refill = ctx->refill;
+    args.chunk = ctx->chunk;
+    hipLaunchKernelGGL((k_trace<LEAF, MODE, STATS, WIDE, MULTI>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st,
+                       args, table);
     RTQ_HIP(ctx, hipGetLastError());
     return RTQ_OK;
 }
 
-template <int LEAF, bool ANY, bool STATS>
-rtq_status launch_layout(rtq_ctx* ctx, const Scene* s, const TraceArgs& a, uint32_t n, bool wide, hipStream_t st) {
-    return wide ? launch<LEAF, ANY, STATS, true>(ctx, s, a, n, st) : launch<LEAF, ANY, STATS, false>(ctx, s, a, n, st);
+// Stack entries a traversal can hold: binary nodes push one entry per tree
+// level below the root, a wide node at most 7 per level.
+uint64_t stack_bound(const Scene* s, bool wide) {
+    return wide ? 7ull * s->wide_depth + 2 : (uint64_t)s->depth + 2;
+}
+
+template <int LEAF, int MODE, bool STATS>
+rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, bool wide, hipStream_t st) {
+    const ItemTable<false> none{};
+    a.nodes = wide ? s->wide : s->nodes;
+    TraceScratch& sc = ctx->scratch;
+    return wide ? launch<LEAF, MODE, STATS, true, false>(ctx, sc, stack_bound(s, true), a, none, n, false, st)
+                : launch<LEAF, MODE, STATS, false, false>(ctx, sc, stack_bound(s, false), a, none, n, false, st);
 }
 
 }  // namespace
 
+rtq_status scratch_alloc(rtq_ctx* ctx, TraceScratch* sc) {
+    RTQ_HIP(ctx, dev_alloc(&sc->counter, 64));
+    RTQ_HIP(ctx, hipMemset(sc->counter, 0, 64 * sizeof(uint32_t)));
+    return RTQ_OK;
+}
+
+void scratch_free(TraceScratch* sc) {
+    if (sc->counter) (void)hipFree(sc->counter);
+    if (sc->spill) (void)hipFree(sc->spill);
+    *sc = TraceScratch{};
+}
+
 rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,
                  uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st) {
     if (n == 0) return RTQ_OK;
-    // the counter runs past n by at most one chunk per wave: n < 2^31 keeps it in range
+    // the counter runs past n by at most one chunk per wave: n <= 2^31 keeps it in range
     if (n > 0x80000000ull) return fail(ctx, RTQ_ECAPACITY, "trace: n_rays must be <= 2^31");
     TraceArgs args{};
     args.tris = s->tris;
     args.rays = rays;
     args.hits = hits;
     args.n = (uint32_t)n;
-    args.refill = ctx->refill;
-    args.chunk = ctx->chunk;
     args.stat_nodes = stat_nodes;
     args.stat_tris = stat_tris;
     const bool any = flags & RTQ_TRACE_ANY;
     const bool refmt = flags & RTQ_TRACE_LEAF_REFMT;
-    const bool wide = s->wide != nullptr && !(flags & RTQ_TRACE_BINARY);
+    const bool wide = s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide;
     const uint32_t nn = (uint32_t)n;
     if (!stats) {
         if (!refmt)
-            return any ? launch_layout<0, true, false>(ctx, s, args, nn, wide, st)
-                       : launch_layout<0, false, false>(ctx, s, args, nn, wide, st);
-        return any ? launch_layout<1, true, false>(ctx, s, args, nn, wide, st)
-                   : launch_layout<1, false, false>(ctx, s, args, nn, wide, st);
+            return any ? launch_single<0, 1, false>(ctx, s, args, nn, wide, st)
+                       : launch_single<0, 0, false>(ctx, s, args, nn, wide, st);
+        return any ? launch_single<1, 1, false>(ctx, s, args, nn, wide, st)
+                   : launch_single<1, 0, false>(ctx, s, args, nn, wide, st);
     }
     if (!refmt)
-        return any ? launch_layout<0, true, true>(ctx, s, args, nn, wide, st)
-                   : launch_layout<0, false, true>(ctx, s, args, nn, wide, st);
-    return any ? launch_layout<1, true, true>(ctx, s, args, nn, wide, st)
-               : launch_layout<1, false, true>(ctx, s, args, nn, wide, st);
+        return any ? launch_single<0, 1, true>(ctx, s, args, nn, wide, st)
+                   : launch_single<0, 0, true>(ctx, s, args, nn, wide, st);
+    return any ? launch_single<1, 1, true>(ctx, s, args, nn, wide, st)
+               : launch_single<1, 0, true>(ctx, s, args, nn, wide, st);
+}
+
+// Up to kMaxBatchItems ray sets (any scenes, same leaf test and layout) in one
+// persistent launch: waves take chunks from one counter over the concatenated
+// ray index space, so uneven sets balance across the whole chip.
+rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
+                       bool size_only, hipStream_t st) {
+    if (n_items == 0) return RTQ_OK;
+    if (!items) return fail(ctx, RTQ_EINVAL, "trace_batch: null item array");
+    if (n_items > kMaxBatchItems) return fail(ctx, RTQ_ECAPACITY, "trace_batch: at most %u items per launch", kMaxBatchItems);
+    ItemTable<true> table{};
+    uint64_t total = 0, depth = 0;
+    const uint32_t leaf_flags = items[0].flags & RTQ_TRACE_LEAF_REFMT;
+    int layout = -1;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const rtq_batch_item& it = items[i];
+        const Scene* s = find_scene(ctx, it.scene_id);
+        if (!s) return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: unknown scene %u", i, it.scene_id);
+        if (!s->built) return fail(ctx, RTQ_ENOTBUILT, "trace_batch: item %u: scene %u has no BVH", i, it.scene_id);
+        if (it.flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT | RTQ_TRACE_BINARY))
+            return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: unknown flags 0x%x", i, it.flags);
+        if ((it.flags & RTQ_TRACE_LEAF_REFMT) != leaf_flags)
+            return fail(ctx, RTQ_EINVAL, "trace_batch: all items must use the same leaf test");
+        if (it.n_rays && (!it.rays_dev || !it.hits_dev))
+            return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: null buffer", i);
+        const bool wide = s->wide != nullptr && !(it.flags & RTQ_TRACE_BINARY) && ctx->prefer_wide;
+        if (layout < 0) layout = wide;
+        if (layout != (int)wide) return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
+        BatchItem& b = table.item[i];
+        b.nodes = wide ? s->wide : s->nodes;
+        b.tris = s->tris;
+        b.rays = it.rays_dev;
+        b.hits = it.hits_dev;
+        b.begin = (uint32_t)total;
+        b.n = (uint32_t)it.n_rays;
+        b.any = (it.flags & RTQ_TRACE_ANY) ? 1u : 0u;
+        total += it.n_rays;
+        if (total > 0x80000000ull) return fail(ctx, RTQ_ECAPACITY, "trace_batch: more than 2^31 rays in one launch");
+        depth = std::max(depth, stack_bound(s, wide));
+    }
+    if (total == 0) return RTQ_OK;
+    TraceArgs args{};
+    args.n = (uint32_t)total;
+    args.n_items = n_items;
+    TraceScratch& sc = scratch ? *scratch : ctx->scratch;
+    const uint32_t nt = (uint32_t)total;
+    if (leaf_flags)
+        return layout ? launch<1, 2, false, true, true>(ctx, sc, depth, args, table, nt, size_only, st)
+                      : launch<1, 2, false, false, true>(ctx, sc, depth, args, table, nt, size_only, st);
+    return layout ? launch<0, 2, false, true, true>(ctx, sc, depth, args, table, nt, size_only, st)
+                  : launch<0, 2, false, false, true>(ctx, sc, depth, args, table, nt, size_only, st);
 }
 
 }  // namespace rtq

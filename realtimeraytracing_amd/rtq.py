@@ -45,7 +45,25 @@ EXPORTED = [
     "rtq_bvh_build", "rtq_bvh_info_get", "rtq_bvh_export_ref", "rtq_bvh_export_clusters",
     "rtq_morton_codes", "rtq_sort_pairs", "rtq_scan_exclusive_u32", "rtq_trace", "rtq_trace_stats",
     "rtq_trace_host", "rtq_raygen_pinhole", "rtq_raygen_diffuse", "rtq_raygen_shadow",
+    "rtq_trace_batch", "rtq_batch_create", "rtq_batch_launch", "rtq_batch_destroy",
+    "rtq_launch_params_get", "rtq_launch_params_set", "rtq_autotune", "rtq_tune_save", "rtq_tune_load",
 ]
+MAX_BATCH_ITEMS = 64
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("scene_id", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("rays_dev", ctypes.c_void_p),
+                ("hits_dev", ctypes.c_void_p), ("n_rays", ctypes.c_uint64)]
+
+
+class LaunchParams(ctypes.Structure):
+    _fields_ = [("refill", ctypes.c_uint32), ("chunk", ctypes.c_uint32), ("prefer_wide", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("best", LaunchParams), ("best_ms", ctypes.c_float), ("default_ms", ctypes.c_float),
+                ("n_tried", ctypes.c_uint32), ("from_cache", ctypes.c_uint32)]
 
 
 class BuildParams(ctypes.Structure):
@@ -110,6 +128,15 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_raygen_pinhole": (st, [vp, ctypes.POINTER(Pinhole), vp, vp]),
         "rtq_raygen_diffuse": (st, [vp, u32, vp, vp, u64, u64, ctypes.c_float, vp, vp]),
         "rtq_raygen_shadow": (st, [vp, u32, vp, vp, u64, vp, u32, ctypes.c_float, vp, vp]),
+        "rtq_trace_batch": (st, [vp, ctypes.POINTER(BatchItem), u32, vp]),
+        "rtq_batch_create": (st, [vp, ctypes.POINTER(BatchItem), u32, ctypes.POINTER(vp)]),
+        "rtq_batch_launch": (st, [vp, vp]),
+        "rtq_batch_destroy": (None, [vp]),
+        "rtq_launch_params_get": (st, [vp, ctypes.POINTER(LaunchParams)]),
+        "rtq_launch_params_set": (st, [vp, ctypes.POINTER(LaunchParams)]),
+        "rtq_autotune": (st, [vp, ctypes.POINTER(BatchItem), u32, u32, ctypes.POINTER(TuneResult)]),
+        "rtq_tune_save": (st, [vp, ctypes.c_char_p]),
+        "rtq_tune_load": (st, [vp, ctypes.c_char_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -171,6 +198,59 @@ class Context:
 
     def raygen_pinhole(self, cam: Pinhole, rays_ptr: int, stream: int | None = None):
         self.check(self.lib.rtq_raygen_pinhole(self.h, ctypes.byref(cam), rays_ptr, stream))
+
+    # batched traversal
+    @staticmethod
+    def items(specs) -> "ctypes.Array":
+        """specs: iterable of (scene, rays_ptr, n_rays, hits_ptr, flags)."""
+        specs = list(specs)
+        arr = (BatchItem * len(specs))()
+        for i, (scene, rays_ptr, n, hits_ptr, flags) in enumerate(specs):
+            arr[i] = BatchItem(scene.id, flags, rays_ptr, hits_ptr, n)
+        return arr
+
+    def trace_batch(self, items, stream: int | None = None) -> None:
+        self.check(self.lib.rtq_trace_batch(self.h, items, len(items), stream))
+
+    def batch(self, items) -> "Batch":
+        h = ctypes.c_void_p()
+        self.check(self.lib.rtq_batch_create(self.h, items, len(items), ctypes.byref(h)))
+        return Batch(self, h, items)
+
+    def launch_params(self) -> LaunchParams:
+        p = LaunchParams()
+        self.check(self.lib.rtq_launch_params_get(self.h, ctypes.byref(p)))
+        return p
+
+    def set_launch_params(self, refill: int, chunk: int, prefer_wide: bool = True) -> None:
+        p = LaunchParams(refill, chunk, int(prefer_wide), 0)
+        self.check(self.lib.rtq_launch_params_set(self.h, ctypes.byref(p)))
+
+    def autotune(self, items, reps: int = 3) -> TuneResult:
+        out = TuneResult()
+        self.check(self.lib.rtq_autotune(self.h, items, len(items), reps, ctypes.byref(out)))
+        return out
+
+    def tune_save(self, path) -> None:
+        self.check(self.lib.rtq_tune_save(self.h, str(path).encode()))
+
+    def tune_load(self, path) -> None:
+        self.check(self.lib.rtq_tune_load(self.h, str(path).encode()))
+
+
+class Batch:
+    """A batch captured in a hipGraph (rtq_batch)."""
+
+    def __init__(self, ctx: Context, h, items):
+        self.ctx, self.h, self.items = ctx, h, items
+
+    def launch(self, stream: int | None = None) -> None:
+        self.ctx.check(self.ctx.lib.rtq_batch_launch(self.h, stream))
+
+    def destroy(self) -> None:
+        if self.h:
+            self.ctx.lib.rtq_batch_destroy(self.h)
+            self.h = None
 
 
 class Scene:

@@ -195,6 +195,52 @@ SECONDARY_EPS = 1e-4
 DIFFUSE_SEED = 7
 
 
+def sphere_with(n_tris: int, seed: int, jitter: float = 1e-3) -> np.ndarray:
+    """A jittered unit sphere with about n_tris triangles (4k^2 - 4k for k rings)."""
+    k = max(2, int(round(math.sqrt(max(n_tris, 8) / 4.0))))
+    return sphere(k, seed=seed, jitter=jitter)
+
+
+def random_rays(n: int, seed: int, lo, hi) -> np.ndarray:
+    """Origins uniform in the box [lo, hi], directions uniform on the sphere."""
+    from .rtq import RAY_DTYPE
+    i = np.arange(n, dtype=np.uint64)
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    o = np.stack([u01(seed, 10 + k, i) for k in range(3)], axis=1) * (hi - lo) + lo
+    # uniform direction: z = 2u - 1, phi = 2 pi v
+    z = 2.0 * u01(seed, 13, i) - 1.0
+    phi = 2.0 * math.pi * u01(seed, 14, i)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    rays = np.zeros(n, RAY_DTYPE)
+    rays["o"] = o.astype(np.float32)
+    rays["d"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def batched_workload(n_scenes: int = 64, seed: int = 0, tri_range=(1_000, 4_000_000), ray_range=(65_536, 2_097_152)):
+    """Uneven scenes for one batched launch: triangle counts log-uniform in
+    tri_range, alternating random soup and jittered spheres, each with a
+    log-uniform number of random-direction rays from inside its bounds.
+    Returns a list of (triangles, rays)."""
+    out = []
+    for s in range(n_scenes):
+        lt = math.log(tri_range[0]) + u01(seed, 20, s) * (math.log(tri_range[1]) - math.log(tri_range[0]))
+        lr = math.log(ray_range[0]) + u01(seed, 21, s) * (math.log(ray_range[1]) - math.log(ray_range[0]))
+        n_tris, n_rays = int(math.exp(lt)), int(math.exp(lr))
+        if s % 2 == 0:
+            tris = soup(n_tris, seed=seed * 1000 + s)
+            lo, hi = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+        else:
+            tris = sphere_with(n_tris, seed=seed * 1000 + s)
+            lo, hi = (-0.9, -0.9, -0.9), (0.9, 0.9, 0.9)
+        out.append((tris, random_rays(n_rays, seed * 1000 + s, lo, hi)))
+    return out
+
+
 def soup_scene(n: int = 65536, seed: int = 0):
     return soup(n, seed), identity_models()
 

@@ -1,0 +1,158 @@
+"""Batched traversal: many scenes and uneven ray sets in one persistent
+launch, its hipGraph form and the autotuner.  Every hit record must be
+bit-identical to tracing each set on its own (and so to the oracle)."""
+import numpy as np
+import pytest
+
+import oracle
+from realtimeraytracing_amd import (HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_ANY, RTQ_TRACE_LEAF_REFMT, Context, RtqError,
+                                    devmem, scenes)
+from conftest import first_diff, hits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+class Workload:
+    def __init__(self, ctx, n_scenes, seed, node_width=2, tri_range=(1_000, 60_000), ray_range=(5_000, 40_000)):
+        self.ctx = ctx
+        self.sets = scenes.batched_workload(n_scenes, seed, tri_range, ray_range)
+        self.scenes, self.d_rays, self.d_hits, self.flags = [], [], [], []
+        for k, (tris, rays) in enumerate(self.sets):
+            sc = ctx.upload(tris, scenes.identity_models())
+            sc.build(16, 4, 0, node_width=node_width)
+            self.scenes.append(sc)
+            self.d_rays.append(devmem.to_device(rays))
+            self.d_hits.append(devmem.empty(len(rays), HIT_DTYPE))
+            self.flags.append(RTQ_TRACE_ANY if k % 3 == 2 else 0)
+
+    def items(self, extra_flags=0):
+        return self.ctx.items((sc, r.data_ptr(), len(s[1]), h.data_ptr(), f | extra_flags)
+                              for sc, r, h, f, s in zip(self.scenes, self.d_rays, self.d_hits, self.flags, self.sets))
+
+    def solo(self, extra_flags=0):
+        out = []
+        for sc, (tris, rays), f in zip(self.scenes, self.sets, self.flags):
+            out.append(sc.trace_host(rays, f | extra_flags))
+        return out
+
+    def hits(self):
+        self.ctx.synchronize()
+        import torch
+        torch.cuda.synchronize()
+        return [devmem.to_host(h, HIT_DTYPE) for h in self.d_hits]
+
+    def release(self):
+        for sc in self.scenes:
+            sc.release()
+
+
+def check_same(got, want, flags):
+    for k, (g, w, f) in enumerate(zip(got, want, flags)):
+        if f & RTQ_TRACE_ANY:
+            assert np.array_equal(g["id"] >= 0, w["id"] >= 0), f"item {k}"
+        else:
+            assert hits_equal(g, w), f"item {k}: {first_diff(g, w)}"
+
+
+@pytest.mark.parametrize("node_width", [2, 8])
+def test_batch_equals_individual_traces(rtq_ctx, node_width):
+    wl = Workload(rtq_ctx, 12, seed=node_width, node_width=node_width)
+    try:
+        rtq_ctx.trace_batch(wl.items())
+        check_same(wl.hits(), wl.solo(), wl.flags)
+        # and the oracle, for a sample of rays of two items
+        for k in (0, 1):
+            tris, rays = wl.sets[k]
+            nodes = wl.scenes[k].export_ref()
+            v9 = oracle.world_vertices(tris, scenes.identity_models())
+            sub = np.ascontiguousarray(rays[:3000])
+            want, _, _ = oracle.trace_bvh(nodes, v9, sub)
+            assert hits_equal(wl.hits()[k][:3000], want)
+    finally:
+        wl.release()
+
+
+def test_batch_refmt_and_errors(rtq_ctx):
+    wl = Workload(rtq_ctx, 4, seed=9)
+    try:
+        rtq_ctx.trace_batch(wl.items(RTQ_TRACE_LEAF_REFMT))
+        check_same(wl.hits(), wl.solo(RTQ_TRACE_LEAF_REFMT), wl.flags)
+        items = wl.items()
+        items[1].flags |= RTQ_TRACE_LEAF_REFMT            # mixed leaf tests
+        with pytest.raises(RtqError):
+            rtq_ctx.trace_batch(items)
+        items = wl.items()
+        items[2].scene_id = 99999                          # unknown scene
+        with pytest.raises(RtqError):
+            rtq_ctx.trace_batch(items)
+        many = rtq_ctx.items([(wl.scenes[0], wl.d_rays[0].data_ptr(), 10, wl.d_hits[0].data_ptr(), 0)] * 65)
+        with pytest.raises(RtqError):
+            rtq_ctx.trace_batch(many)
+    finally:
+        wl.release()
+
+
+def test_graph_replays(rtq_ctx, torch_mod):
+    torch = torch_mod
+    wl = Workload(rtq_ctx, 10, seed=3)
+    try:
+        want = wl.solo()
+        b = rtq_ctx.batch(wl.items())
+        try:
+            stream = torch.cuda.Stream()
+            for rep in range(4):
+                with torch.cuda.stream(stream):
+                    for h in wl.d_hits:
+                        h.zero_()              # eager work between replays
+                b.launch(stream.cuda_stream)
+                stream.synchronize()
+                check_same(wl.hits(), want, wl.flags)
+                # an eager trace of another scene between replays
+                one = rtq_ctx.items([(wl.scenes[0], wl.d_rays[0].data_ptr(), len(wl.sets[0][1]),
+                                      wl.d_hits[0].data_ptr(), wl.flags[0])])
+                rtq_ctx.trace_batch(one)
+                rtq_ctx.synchronize()
+        finally:
+            b.destroy()
+    finally:
+        wl.release()
+
+
+def test_autotune_cache_roundtrip(rtq_ctx, tmp_path):
+    wl = Workload(rtq_ctx, 6, seed=5, node_width=8)
+    try:
+        before = rtq_ctx.launch_params()
+        res = rtq_ctx.autotune(wl.items(), reps=1)
+        assert res.n_tried > 10 and res.from_cache == 0
+        assert res.best_ms <= res.default_ms
+        got = rtq_ctx.launch_params()
+        assert (got.refill, got.chunk, got.prefer_wide) == (res.best.refill, res.best.chunk, res.best.prefer_wide)
+        check_same(wl.hits(), wl.solo(), wl.flags)         # tuned settings give the same hits
+        again = rtq_ctx.autotune(wl.items(), reps=1)
+        assert again.from_cache == 1 and again.n_tried == 0
+        path = tmp_path / "tune.bin"
+        rtq_ctx.tune_save(path)
+        with Context(0) as other:
+            other.tune_load(path)
+            # the key depends on scenes of this context: same sizes -> same key
+            ws = [(sc, r.data_ptr(), len(s[1]), h.data_ptr(), f)
+                  for sc, r, h, f, s in zip(wl.scenes, wl.d_rays, wl.d_hits, wl.flags, wl.sets)]
+            assert len(ws) == 6
+        bad = tmp_path / "bad.bin"
+        data = bytearray(path.read_bytes())
+        data[-1] ^= 0xFF
+        bad.write_bytes(bytes(data))
+        with pytest.raises(RtqError):
+            rtq_ctx.tune_load(bad)
+        rtq_ctx.set_launch_params(before.refill, before.chunk, bool(before.prefer_wide))
+    finally:
+        wl.release()
+
+
+def test_launch_params_validation(rtq_ctx):
+    p = rtq_ctx.launch_params()
+    with pytest.raises(RtqError):
+        rtq_ctx.set_launch_params(0, 128)
+    with pytest.raises(RtqError):
+        rtq_ctx.set_launch_params(8, 5000)
+    rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide))

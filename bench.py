@@ -11,9 +11,12 @@ resident in HBM.  Mrays/s counts only rays with a non-empty interval (every
 primary ray hits, so all 10,368,000 are live), M = 10^6.
 
 Forms measured on the same scene and rays (hit records are bit-identical
-between them, checked in tests/test_gpu_wide.py):
-  plain       binary 64-B nodes
+between them, checked in tests/test_gpu_wide.py and tests/test_gpu_batch.py):
+  plain       binary 64-B nodes, one launch per ray set, default settings
   compressed  8-wide 128-B nodes with 8-bit outward-rounded child boxes
+  batched     both ray sets in one persistent launch replayed from a
+              hipGraph, settings (refill, chunk, node layout) from the
+              autotuner
 `value` is the fastest form; `detail.forms` has every form.
 
 Prints one JSON line with the metric, a roofline object for the dominant
@@ -39,7 +42,6 @@ HBM_PEAK_GBS = 8000.0
 TRI_BYTES = 48       # leaf triangle record
 RAY_BYTES = 32
 HIT_BYTES = 16
-FORMS = {"plain": 64, "compressed": 128}   # node bytes per visit
 
 
 def parse():
@@ -47,7 +49,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--forms", default="plain,compressed")
+    ap.add_argument("--forms", default="plain,compressed,batched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -110,11 +112,8 @@ def main() -> int:
     live_rays = n_live + 4 * n_live
     setup_s = time.time() - t0
 
-    results = {}
-    for form in args.forms.split(","):
-        layout = RTQ_TRACE_BINARY if form == "plain" else 0
-        node_bytes = FORMS[form]
-        # traversal counters (outside the timed region): algorithmic bytes per launch
+    # traversal counters of each layout (outside the timed region): algorithmic bytes per launch
+    def layout_counters(layout, node_bytes):
         counters = {}
         for name, d_rays, cnt, d_hits, flags in (("diffuse", d_diff, n_diff, d_hd, 0),
                                                   ("shadow", d_shad, n_shad, d_hs, RTQ_TRACE_ANY)):
@@ -126,17 +125,12 @@ def main() -> int:
             t = int(devmem.to_host(dt, np.uint32).astype(np.int64).sum())
             counters[name] = dict(rays=cnt, node_visits=v, tri_tests=t,
                                   bytes=cnt * (RAY_BYTES + HIT_BYTES) + v * node_bytes + t * TRI_BYTES)
+        return counters
 
-        def step(ev=None):
-            if ev is not None:
-                ev[0].record(stream)
-            sc.trace(d_diff.data_ptr(), n_diff, d_hd.data_ptr(), layout, sp)
-            if ev is not None:
-                ev[1].record(stream)
-            sc.trace(d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY | layout, sp)
-            if ev is not None:
-                ev[2].record(stream)
+    default_params = ctx.launch_params()
+    counters_by_layout = {"binary": layout_counters(RTQ_TRACE_BINARY, 64), "wide": layout_counters(0, 128)}
 
+    def time_steps(step):
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
@@ -147,24 +141,76 @@ def main() -> int:
             step(evs[k])
         stop.record(stream)
         torch.cuda.synchronize()
-        ms_per_step = start.elapsed_time(stop) / args.steps
-        diff_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-        shad_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-        dom = "shadow" if shad_ms >= diff_ms else "diffuse"
-        dom_ms = shad_ms if dom == "shadow" else diff_ms
-        achieved = counters[dom]["bytes"] / (dom_ms * 1e-3) / 1e9
-        kernel = f"k_trace<watertight,{'any' if dom == 'shadow' else 'closest'},{form}>"
+        return start.elapsed_time(stop) / args.steps, evs
+
+    results = {}
+    for form in args.forms.split(","):
+        ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide))
+        tuned = None
+        if form in ("plain", "compressed"):
+            layout = RTQ_TRACE_BINARY if form == "plain" else 0
+            lname = "binary" if form == "plain" else "wide"
+
+            def step(ev=None):
+                if ev is not None:
+                    ev[0].record(stream)
+                sc.trace(d_diff.data_ptr(), n_diff, d_hd.data_ptr(), layout, sp)
+                if ev is not None:
+                    ev[1].record(stream)
+                sc.trace(d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY | layout, sp)
+                if ev is not None:
+                    ev[2].record(stream)
+
+            ms_per_step, evs = time_steps(step)
+            diff_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+            shad_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+            counters = counters_by_layout[lname]
+            dom = "shadow" if shad_ms >= diff_ms else "diffuse"
+            dom_ms = shad_ms if dom == "shadow" else diff_ms
+            dom_bytes = counters[dom]["bytes"]
+            kernel = f"k_trace<watertight,{'any' if dom == 'shadow' else 'closest'},{lname}>"
+        elif form == "batched":
+            # both ray sets in one persistent launch, replayed from a hipGraph,
+            # launch settings (incl. node layout) from the autotuner
+            items = ctx.items([(sc, d_diff.data_ptr(), n_diff, d_hd.data_ptr(), 0),
+                               (sc, d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY)])
+            tr = ctx.autotune(items, reps=2)
+            tuned = {"refill": tr.best.refill, "chunk": tr.best.chunk, "prefer_wide": tr.best.prefer_wide,
+                     "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4), "default_ms": round(tr.default_ms, 4)}
+            lname = "wide" if tr.best.prefer_wide else "binary"
+            graph = ctx.batch(items)
+
+            def step(ev=None):
+                if ev is not None:
+                    ev[0].record(stream)
+                graph.launch(sp)
+                if ev is not None:
+                    ev[1].record(stream)
+                    ev[2].record(stream)
+
+            ms_per_step, evs = time_steps(step)
+            graph.destroy()
+            diff_ms = shad_ms = None
+            counters = counters_by_layout[lname]
+            dom = "batch"
+            dom_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+            dom_bytes = counters["diffuse"]["bytes"] + counters["shadow"]["bytes"]
+            kernel = f"k_trace<watertight,per-item,{lname},batch>"
+        else:
+            raise SystemExit(f"unknown form {form}")
+        achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+        node_bytes = 64 if lname == "binary" else 128
         results[form] = {
             "Mrays_s": live_rays / (ms_per_step * 1e-3) / 1e6,
             "ms_per_step": ms_per_step, "diffuse_ms": diff_ms, "shadow_ms": shad_ms,
-            "diffuse_Mrays_s": n_live / (diff_ms * 1e-3) / 1e6, "shadow_Mrays_s": 4 * n_live / (shad_ms * 1e-3) / 1e6,
-            "counters": counters,
+            "layout": lname, "tuned": tuned, "counters": counters,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.traffic_file, kernel),
                          "kernel": kernel, "launch_ms": round(dom_ms, 4),
-                         "algorithmic_bytes_per_launch": counters[dom]["bytes"],
+                         "algorithmic_bytes_per_launch": dom_bytes,
                          "bytes_per_ray": "32 ray + 16 hit + %d x node visits + 48 x triangle tests" % node_bytes},
         }
+    ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide))
 
     best = max(results, key=lambda f: results[f]["Mrays_s"])
     res = results[best]

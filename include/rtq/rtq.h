@@ -120,6 +120,18 @@ rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out);
 rtq_status rtq_bvh_export_ref(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_node* nodes,
                               uint64_t capacity, uint64_t* n_nodes);
 
+/* The binary traversal layout (no counterpart in the reference, whose shader
+ * reads the BVH_NodeGPU array directly, raytracer.glsl:246-295): 64-B nodes,
+ *   [c0.lo.x c0.hi.x c0.lo.y c0.hi.y] [c0.lo.z c0.hi.z c1.lo.x c1.hi.x]
+ *   [c1.lo.y c1.hi.y c1.lo.z c1.hi.z] [ref0 ref1 order 0]
+ * (boxes widened by the build's pad; ref bit 31 set = leaf with first
+ * leaf-order triangle in bits 30..3 and count - 1 in bits 2..0, else a node
+ * index; order = learned order word), and per node 4 u32: the leaf-order
+ * ranges of its children, [r0, r1) and [r1, r2), and its depth.  Either
+ * buffer may be NULL; *n_nodes receives the node count. */
+rtq_status rtq_bvh_export_trav(rtq_ctx* ctx, uint32_t scene_id, void* nodes, uint32_t* ranges,
+                               uint64_t capacity, uint64_t* n_nodes);
+
 /* The build's cluster arrays in the form of BVH_Params (bvh.hpp:44-63):
  * cluster c < N is the leaf for sorted position c, c >= N are merges in the
  * order they were made (ids assigned in increasing position order within a
@@ -158,6 +170,9 @@ rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t
                                        default leaf test is watertight FP32 */
 #define RTQ_TRACE_BINARY      0x4u  /* trace the binary layout even if the scene was
                                        built with node_width 8 */
+#define RTQ_TRACE_LEARNED     0x8u  /* binary layout, child order at nodes where both
+                                       children are hit taken from the order bits that
+                                       rtq_order_apply stored (hits are unchanged) */
 
 /* Trace n_rays rays against the scene's BVH.  Replaces the traversal of
  * getClosestHitBVH (raytracer.glsl:246-295) dispatched by drawOneFrame
@@ -271,6 +286,81 @@ rtq_status rtq_raygen_diffuse(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* pr
 rtq_status rtq_raygen_shadow(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* prim_dev,
                              const rtq_hit* hits_dev, uint64_t n, const float* lights,
                              uint32_t n_lights, float eps, rtq_ray* out_dev, void* stream);
+
+/* ---- learned child ordering (no counterpart in the reference: its
+ * traversal always enters the left child first, raytracer.glsl:264-290) ----
+ *
+ * A 32 -> 64 -> 64 -> 1 MLP (ReLU, ReLU, logistic) scores a binary node for a
+ * ray octant (bit k set <=> direction component k < 0) and a mode (closest /
+ * any hit): the probability that entering child 0 first is the better order.
+ * Parameters: RTQ_ORDER_PARAMS floats, W1 [32][64] (input-major), b1 [64],
+ * W2 [64][64], b2 [64], W3 [64], b3.  A sample is RTQ_ORDER_SAMPLE floats:
+ * 32 features, the label (1 = child 0 first) and a loss weight. */
+#define RTQ_ORDER_INPUTS 32
+#define RTQ_ORDER_HIDDEN 64
+#define RTQ_ORDER_PARAMS 6337
+#define RTQ_ORDER_SAMPLE 34
+
+/* Traversal logs from every `stride`-th ray (closest hit, or any hit with
+ * RTQ_TRACE_ANY).  From the root, each node where both children are hit is
+ * priced in both visit orders by sub-traversals (nearer child first inside
+ * them; node visits + triangle tests until the closest hit is certain, or
+ * until an occluder is found); the sample's label is the cheaper child first
+ * and its weight log2(1 + cost difference) (ties give no sample).  The walk
+ * continues into the cheaper child and stops where no hit lies below it.
+ * At most `capacity` samples are written (in no particular order); *n_samples
+ * receives the count written.  Synchronous. */
+rtq_status rtq_order_log(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays,
+                         uint32_t flags, uint32_t stride, float* samples_dev, uint64_t capacity,
+                         uint64_t* n_samples);
+
+/* He-uniform initial parameters from `seed` (Adam moments are the caller's
+ * zeroed buffers). */
+rtq_status rtq_order_init(rtq_ctx* ctx, uint64_t seed, float* params_dev);
+
+/* Gradient of the mean weighted binary cross-entropy over samples [0, n)
+ * (FP32, on the GPU); *loss (may be NULL) receives the mean loss.  Synchronous. */
+rtq_status rtq_order_grad(rtq_ctx* ctx, const float* params_dev, const float* samples_dev, uint32_t n,
+                          float* grad_dev, float* loss);
+
+typedef struct rtq_train_params {
+    uint32_t epochs;
+    uint32_t batch;        /* samples per Adam step: a multiple of 64, at most 65536 */
+    float lr;
+    float beta1;
+    float beta2;
+    uint32_t reserved;
+    uint64_t seed;         /* per-epoch sample permutation */
+} rtq_train_params;
+
+/* Adam on the GPU over `epochs` passes of the samples; *step counts Adam steps
+ * (bias correction) across calls; epoch_loss (may be NULL) receives the mean
+ * loss of each epoch, measured during its steps.  Synchronous. */
+rtq_status rtq_order_train(rtq_ctx* ctx, const float* samples_dev, uint64_t n_samples,
+                           const rtq_train_params* tp, float* params_dev, float* adam_m_dev,
+                           float* adam_v_dev, uint32_t* step, float* epoch_loss);
+
+/* Inference with BF16 MFMA tiles (FP32 accumulation): probabilities of n rows
+ * of 32 features, `stride` floats apart.  Synchronous. */
+rtq_status rtq_order_predict(rtq_ctx* ctx, const float* params_dev, const float* features_dev, uint64_t n,
+                             uint32_t stride, float* prob_dev);
+
+/* Scores every binary node of the scene for the 8 octants x 2 modes with the
+ * MFMA inference and stores the decisions in the node; RTQ_TRACE_LEARNED then
+ * takes the predicted child first where max(p, 1 - p) reaches the mode's
+ * confidence (>= 0.5; >= 1 turns the mode's decisions off) and the nearer
+ * child elsewhere (also where p is NaN).  params_dev NULL clears the
+ * decisions (RTQ_TRACE_LEARNED is then refused).  Rebuilding the BVH clears
+ * them too.  Synchronous. */
+rtq_status rtq_order_apply(rtq_ctx* ctx, uint32_t scene_id, const float* params_dev, float confidence_closest,
+                           float confidence_any);
+
+/* Checkpoint: parameters, Adam moments and step in a versioned, checksummed
+ * file (magic "RTOM", version 1). */
+rtq_status rtq_order_save(rtq_ctx* ctx, const char* path, const float* params_dev, const float* adam_m_dev,
+                          const float* adam_v_dev, uint32_t step);
+rtq_status rtq_order_load(rtq_ctx* ctx, const char* path, float* params_dev, float* adam_m_dev,
+                          float* adam_v_dev, uint32_t* step);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop

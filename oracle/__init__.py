@@ -66,6 +66,12 @@ def lib() -> ctypes.CDLL:
             "orc_raygen_shadow": (None, [vp, vp, vp, u64, vp, u32, f32, vp]),
             "orc_scan_exclusive": (None, [vp, vp, u64]),
             "orc_surface_area": (f32, [vp]),
+            "orc_mlp_param_count": (u32, []),
+            "orc_bf16_round": (f32, [f32]),
+            "orc_mlp_logits": (None, [vp, vp, u64, u32, i32, vp]),
+            "orc_mlp_loss": (ctypes.c_double, [vp, vp, u64]),
+            "orc_mlp_grad": (ctypes.c_double, [vp, vp, u64, vp]),
+            "orc_order_features": (None, [vp, vp, u32, u32, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -237,3 +243,50 @@ def scan_exclusive(a) -> np.ndarray:
     out = np.zeros_like(a)
     lib().orc_scan_exclusive(_p(a), _p(out), len(a))
     return out
+
+
+# ---- learned child ordering (order.c) ----------------------------------------
+
+MLP_INPUTS, MLP_HIDDEN, MLP_SAMPLE = 32, 64, 34
+
+
+def mlp_param_count() -> int:
+    return lib().orc_mlp_param_count()
+
+
+def bf16_round(x: float) -> float:
+    return lib().orc_bf16_round(x)
+
+
+def mlp_logits(params, x, bf16=False) -> np.ndarray:
+    """FP64 logits of the rows of x (n x >= 32 floats; the first 32 are used)."""
+    P = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(len(X), np.float64)
+    lib().orc_mlp_logits(_p(P), _p(X), len(X), X.shape[1], int(bf16), _p(out))
+    return out
+
+
+def mlp_loss(params, samples) -> float:
+    P = np.ascontiguousarray(params, np.float32)
+    S = np.ascontiguousarray(samples, np.float32)
+    assert S.shape[1] == MLP_SAMPLE
+    return lib().orc_mlp_loss(_p(P), _p(S), len(S))
+
+
+def mlp_grad(params, samples):
+    """(mean weighted BCE, FP64 gradient)."""
+    P = np.ascontiguousarray(params, np.float32)
+    S = np.ascontiguousarray(samples, np.float32)
+    assert S.shape[1] == MLP_SAMPLE
+    g = np.zeros(mlp_param_count(), np.float64)
+    loss = lib().orc_mlp_grad(_p(P), _p(S), len(S), _p(g))
+    return loss, g
+
+
+def order_features(node16, aux4, octant, any_hit) -> np.ndarray:
+    nd = np.ascontiguousarray(node16, np.float32).reshape(16)
+    ax = np.ascontiguousarray(aux4, np.uint32).reshape(4)
+    f = np.zeros(32, np.float32)
+    lib().orc_order_features(_p(nd), _p(ax), int(octant), int(any_hit), _p(f))
+    return f

@@ -25,6 +25,10 @@
  *   orc_trace_ref / orc_trace_bvh    equality with orc_trace_brute
  *   orc_raygen_*                     closed forms (centre pixel, unit length,
  *                                    cosine-weighted moments)
+ *   orc_mlp_grad                     central finite differences of
+ *                                    orc_mlp_loss; all-zero network closed form
+ *   orc_mlp_logits                   hand-built networks with known outputs
+ *   orc_order_features               hand-computed features of a hand node
  */
 #ifndef RTQ_ORACLE_H
 #define RTQ_ORACLE_H
@@ -140,6 +144,22 @@ void orc_raygen_shadow(const float* verts9, const rtq_ray* prim, const rtq_hit* 
 
 /* Exclusive scan (wrapping u32). */
 void orc_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n);
+
+/* ---- learned child ordering (order.c; no reference counterpart) ---------- */
+/* Parameters: W1 [32][64] (input-major), b1 [64], W2 [64][64], b2, W3 [64], b3.
+ * Samples: 32 features, label, weight (34 floats). */
+uint32_t orc_mlp_param_count(void);
+float orc_bf16_round(float f);
+/* Logits of n rows of 32 features `stride` floats apart, in FP64; bf16 != 0
+ * rounds W1, W2, the inputs and the layer-1 activations to BF16 first (the
+ * rounding points of the GPU's MFMA inference). */
+void orc_mlp_logits(const float* params, const float* x, uint64_t n, uint32_t stride, int bf16, double* out);
+/* Mean weighted binary cross-entropy over n samples, FP64. */
+double orc_mlp_loss(const float* params, const float* samples, uint64_t n);
+/* Its gradient (orc_mlp_param_count() doubles); returns the loss. */
+double orc_mlp_grad(const float* params, const float* samples, uint64_t n, double* grad);
+/* Features of one 64-B traversal node for (octant, mode); see order.c. */
+void orc_order_features(const float* node16, const uint32_t* aux4, uint32_t octant, uint32_t any, float* f32);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "librtq.so"
-SOURCES = ["api.hip", "batch.hip", "build.hip", "primitives.hip", "trace.hip", "wide.hip"]
+SOURCES = ["api.hip", "batch.hip", "build.hip", "learned.hip", "primitives.hip", "trace.hip", "wide.hip"]
 ARCH = os.environ.get("RTQ_OFFLOAD_ARCH", "gfx950")
 
 

@@ -58,9 +58,11 @@ void free_build(Scene* s) {
     release(s->sorted_codes);
     release(s->ref_nodes);
     release(s->nodes);
+    release(s->node_aux);
     release(s->tris);
     release(s->wide);
     s->built = false;
+    s->ordered = false;
     s->n_trav = 0;
     s->n_wide = 0;
     s->wide_depth = 0;
@@ -416,6 +418,24 @@ rtq_status rtq_bvh_export_ref(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_node* nod
     return RTQ_OK;
 }
 
+rtq_status rtq_bvh_export_trav(rtq_ctx* ctx, uint32_t scene_id, void* nodes, uint32_t* ranges, uint64_t capacity,
+                               uint64_t* n_nodes) {
+    if (!ctx) return RTQ_EINVAL;
+    Scene* s = find_scene(ctx, scene_id);
+    if (!s) return fail(ctx, RTQ_EINVAL, "bvh_export_trav: unknown scene %u", scene_id);
+    if (!s->built) return fail(ctx, RTQ_ENOTBUILT, "bvh_export_trav: scene %u has no BVH", scene_id);
+    if (n_nodes) *n_nodes = s->n_trav;
+    if (!nodes && !ranges) return RTQ_OK;
+    if (capacity < s->n_trav) return fail(ctx, RTQ_ECAPACITY, "bvh_export_trav: need %u nodes", s->n_trav);
+    RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    if (nodes)
+        RTQ_HIP(ctx, hipMemcpyAsync(nodes, s->nodes, 4 * sizeof(float4) * s->n_trav, hipMemcpyDeviceToHost, ctx->stream));
+    if (ranges)
+        RTQ_HIP(ctx, hipMemcpyAsync(ranges, s->node_aux, sizeof(uint4) * s->n_trav, hipMemcpyDeviceToHost, ctx->stream));
+    RTQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RTQ_OK;
+}
+
 rtq_status rtq_bvh_export_clusters(rtq_ctx* ctx, uint32_t scene_id, uint64_t capacity, rtq_aabb* boxes,
                                    uint32_t* triangle_id, uint32_t* left, uint32_t* right, uint32_t* parent,
                                    uint32_t* sorted_indices, uint32_t* sorted_codes) {
@@ -479,7 +499,7 @@ static rtq_status trace_common(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* r
     Scene* s = find_scene(ctx, scene_id);
     if (!s) return fail(ctx, RTQ_EINVAL, "trace: unknown scene %u", scene_id);
     if (!s->built) return fail(ctx, RTQ_ENOTBUILT, "trace: scene %u has no BVH", scene_id);
-    if (flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT | RTQ_TRACE_BINARY))
+    if (flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT | RTQ_TRACE_BINARY | RTQ_TRACE_LEARNED))
         return fail(ctx, RTQ_EINVAL, "trace: unknown flags 0x%x", flags);
     if (n == 0) return RTQ_OK;
     if (!rays || !hits) return fail(ctx, RTQ_EINVAL, "trace: null ray or hit buffer");

@@ -393,7 +393,8 @@ __global__ __launch_bounds__(kThreads) void k_mark_trav(uint32_t n_clusters, uin
 
 // Traversal node (64 B): child boxes widened outward by `pad`, interleaved as
 //   [c0.lo.x c0.hi.x c0.lo.y c0.hi.y] [c0.lo.z c0.hi.z c1.lo.x c1.hi.x]
-//   [c1.lo.y c1.hi.y c1.lo.z c1.hi.z] [ref0 ref1 0 0]
+//   [c1.lo.y c1.hi.y c1.lo.z c1.hi.z] [ref0 ref1 order 0]
+// where `order` holds the learned child-order bits (zero until a model is applied).
 __device__ __forceinline__ void child_desc(uint32_t ch, uint32_t root, uint32_t max_leaf,
                                            const rtq_aabb* cbox, const uint32_t* cleft,
                                            const uint32_t* ccount, const uint32_t* pos,
@@ -409,7 +410,8 @@ __global__ __launch_bounds__(kThreads) void k_write_trav(
     uint32_t n_clusters, uint32_t root, uint32_t max_leaf, float pad, const rtq_aabb* __restrict__ cbox,
     const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
     const uint32_t* __restrict__ ccount, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ fl,
-    const uint32_t* __restrict__ bidx, float4* __restrict__ nodes) {
+    const uint32_t* __restrict__ bidx, const uint32_t* __restrict__ dep, float4* __restrict__ nodes,
+    uint4* __restrict__ aux) {
     const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
     if (c >= n_clusters) return;
     if (!is_trav_node(c, root, max_leaf, cleft, ccount)) return;
@@ -417,15 +419,21 @@ __global__ __launch_bounds__(kThreads) void k_write_trav(
     uint32_t r0, r1;
     child_desc(cleft[c], root, max_leaf, cbox, cleft, ccount, pos, fl, bidx, pad, l0, h0, &r0);
     child_desc(cright[c], root, max_leaf, cbox, cleft, ccount, pos, fl, bidx, pad, l1, h1, &r1);
-    float4* o = nodes + 4ull * bidx[pos[c]];
+    const uint32_t idx = bidx[pos[c]];
+    float4* o = nodes + 4ull * idx;
     o[0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
     o[1] = make_float4(l0[2], h0[2], l1[0], h1[0]);
     o[2] = make_float4(l1[1], h1[1], l1[2], h1[2]);
+    // .z holds the learned child-order bits (none until an order model is applied)
     o[3] = make_float4(__uint_as_float(r0), __uint_as_float(r1), 0.0f, 0.0f);
+    // leaf-order ranges of the children, [A, B) and [B, C), and the binary depth
+    const uint32_t a0 = fl[cleft[c]], b0 = fl[cright[c]];
+    aux[idx] = make_uint4(a0, b0, b0 + ccount[cright[c]], dep[c]);
 }
 
 // Single-triangle scene: one node whose two children are the same leaf.
-__global__ void k_write_single(const rtq_aabb* __restrict__ cbox, float pad, float4* __restrict__ nodes) {
+__global__ void k_write_single(const rtq_aabb* __restrict__ cbox, float pad, float4* __restrict__ nodes,
+                               uint4* __restrict__ aux) {
     const rtq_aabb a = cbox[0];
     float lo[3], hi[3];
     for (int k = 0; k < 3; ++k) { lo[k] = a.min[k] - pad; hi[k] = a.max[k] + pad; }
@@ -434,6 +442,7 @@ __global__ void k_write_single(const rtq_aabb* __restrict__ cbox, float pad, flo
     nodes[1] = make_float4(lo[2], hi[2], lo[0], hi[0]);
     nodes[2] = make_float4(lo[1], hi[1], lo[2], hi[2]);
     nodes[3] = make_float4(__uint_as_float(r), __uint_as_float(r), 0.0f, 0.0f);
+    aux[0] = make_uint4(0u, 1u, 1u, 0u);
 }
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
@@ -604,7 +613,8 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     if (n == 1) {
         s->n_trav = 1;
         RTQ_HIP(ctx, dev_alloc(&s->nodes, 4));
-        hipLaunchKernelGGL(k_write_single, dim3(1), dim3(1), 0, st, s->cbox, s->box_pad, s->nodes);
+        RTQ_HIP(ctx, dev_alloc(&s->node_aux, 1));
+        hipLaunchKernelGGL(k_write_single, dim3(1), dim3(1), 0, st, s->cbox, s->box_pad, s->nodes, s->node_aux);
         RTQ_HIP(ctx, hipGetLastError());
     } else {
         hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
@@ -617,8 +627,10 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
         RTQ_HIP(ctx, hipStreamSynchronize(st));
         s->n_trav = h_ntrav;
         RTQ_HIP(ctx, dev_alloc(&s->nodes, 4ull * h_ntrav));
+        RTQ_HIP(ctx, dev_alloc(&s->node_aux, h_ntrav));
         hipLaunchKernelGGL(k_write_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
-                           s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, s->nodes);
+                           s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, dep, s->nodes,
+                           s->node_aux);
         RTQ_HIP(ctx, hipGetLastError());
     }
     if (prm.node_width == 8) {

@@ -181,9 +181,16 @@ __device__ __forceinline__ uint32_t slot_ref(uint4 a, uint4 b, uint32_t s) {
 __device__ __forceinline__ bool is_leaf_ref(uint32_t r) { return (r & kLeafBit) && r != kEmpty; }
 __device__ __forceinline__ bool is_node_ref(uint32_t r) { return !(r & kLeafBit); }
 
+// Node layouts a launch traces.
+enum Layout : int {
+    kBinary = 0,   // 64-B two-child nodes, nearer child first
+    kWide = 1,     // 128-B compressed 8-wide nodes
+    kOrdered = 2,  // 64-B two-child nodes, learned child order where both are hit
+};
+
 // MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: rays come from
 // several items (scenes), each lane takes its scene and buffers from its item.
-template <int LEAF, int MODE, bool STATS, bool WIDE, bool MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<MULTI> table) {
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
@@ -202,6 +209,8 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
     rtq_hit* hits_p = a.hits;
     uint32_t local = 0;              // ray index within its ray set
     bool lane_any = MODE == 1;
+    // learned order: bit (octant + 8 * any) of a node's order word set = child 0 first
+    uint32_t order_bit = 0;
 
     int rid = -1;
     // wave-uniform ray pool [pool_next, pool_end); dry once the counter passed n
@@ -262,7 +271,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
     };
     // One internal-node visit: cur <- nearest hit child (farther ones pushed) or a pop.
     auto visit = [&]() {
-        if constexpr (WIDE) {
+        if constexpr (LAYOUT == kWide) {
             const float4* nd = nodes_p + 8ull * cur;
             const float4 h = nd[0];
             const uint4 qx = bits4(nd[1]), qy = bits4(nd[2]), qz = bits4(nd[3]);
@@ -338,7 +347,14 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
             const bool h0 = tn0 <= tf0, h1 = tn1 <= tf1;
             const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
             if (h0 && h1) {
-                const bool first0 = tn0 <= tn1;
+                bool first0;
+                if constexpr (LAYOUT == kOrdered) {
+                    // learned decision where its confidence bit is set, else the nearer child
+                    const uint32_t ow = __float_as_uint(n3.z) >> order_bit;
+                    first0 = ((ow >> 16) & 1u) ? (ow & 1u) != 0 : tn0 <= tn1;
+                } else {
+                    first0 = tn0 <= tn1;
+                }
                 push(first0 ? r1 : r0);
                 cur = first0 ? r0 : r1;
             } else if (h0 || h1) {
@@ -396,6 +412,11 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
                     const float4 r0 = rp[0], r1 = rp[1];
                     ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
                     dx = r1.x; dy = r1.y; dz = r1.z; tbest = r1.w;
+                    if constexpr (LAYOUT == kOrdered) {
+                        const bool any_ray = MODE == 1 || (MODE == 2 && lane_any);
+                        order_bit = (dx < 0.0f ? 1u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 4u : 0u) |
+                                    (any_ray ? 8u : 0u);
+                    }
                     best_id = -1;
                     n_nodes = 0;
                     n_tests = 0;
@@ -514,11 +535,11 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
 
 // Persistent grid sized from occupancy, spill column sized from the stack
 // bound `depth_entries`, one launch (or only the scratch sizing).
-template <int LEAF, int MODE, bool STATS, bool WIDE, bool MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
 rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const TraceArgs& base,
                   const ItemTable<MULTI>& table, uint32_t n, bool size_only, hipStream_t st) {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, MODE, STATS, WIDE, MULTI>,
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>,
                                                                kTraceThreads, 0);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
@@ -548,7 +569,7 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
     args.spill_stride = (uint32_t)threads;
     args.refill = ctx->refill;
     args.chunk = ctx->chunk;
-    hipLaunchKernelGGL((k_trace<LEAF, MODE, STATS, WIDE, MULTI>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st,
+    hipLaunchKernelGGL((k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st,
                        args, table);
     RTQ_HIP(ctx, hipGetLastError());
     return RTQ_OK;
@@ -561,12 +582,28 @@ uint64_t stack_bound(const Scene* s, bool wide) {
 }
 
 template <int LEAF, int MODE, bool STATS>
-rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, bool wide, hipStream_t st) {
+rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, int layout, hipStream_t st) {
     const ItemTable<false> none{};
+    const bool wide = layout == kWide;
     a.nodes = wide ? s->wide : s->nodes;
     TraceScratch& sc = ctx->scratch;
-    return wide ? launch<LEAF, MODE, STATS, true, false>(ctx, sc, stack_bound(s, true), a, none, n, false, st)
-                : launch<LEAF, MODE, STATS, false, false>(ctx, sc, stack_bound(s, false), a, none, n, false, st);
+    const uint64_t depth = stack_bound(s, wide);
+    if (layout == kWide) return launch<LEAF, MODE, STATS, kWide, false>(ctx, sc, depth, a, none, n, false, st);
+    if (layout == kOrdered) return launch<LEAF, MODE, STATS, kOrdered, false>(ctx, sc, depth, a, none, n, false, st);
+    return launch<LEAF, MODE, STATS, kBinary, false>(ctx, sc, depth, a, none, n, false, st);
+}
+
+// Layout traced for a scene and trace flags; kNone-like -1 with an error set
+// when RTQ_TRACE_LEARNED asks for order bits the scene does not hold.
+int pick_layout(rtq_ctx* ctx, const Scene* s, uint32_t flags) {
+    if (flags & RTQ_TRACE_LEARNED) {
+        if (!s->ordered) {
+            fail(ctx, RTQ_ENOTBUILT, "trace: RTQ_TRACE_LEARNED needs order bits (rtq_order_apply)");
+            return -1;
+        }
+        return kOrdered;
+    }
+    return (s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide) ? kWide : kBinary;
 }
 
 }  // namespace
@@ -597,20 +634,21 @@ rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, 
     args.stat_tris = stat_tris;
     const bool any = flags & RTQ_TRACE_ANY;
     const bool refmt = flags & RTQ_TRACE_LEAF_REFMT;
-    const bool wide = s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide;
+    const int layout = pick_layout(ctx, s, flags);
+    if (layout < 0) return RTQ_ENOTBUILT;
     const uint32_t nn = (uint32_t)n;
     if (!stats) {
         if (!refmt)
-            return any ? launch_single<0, 1, false>(ctx, s, args, nn, wide, st)
-                       : launch_single<0, 0, false>(ctx, s, args, nn, wide, st);
-        return any ? launch_single<1, 1, false>(ctx, s, args, nn, wide, st)
-                   : launch_single<1, 0, false>(ctx, s, args, nn, wide, st);
+            return any ? launch_single<0, 1, false>(ctx, s, args, nn, layout, st)
+                       : launch_single<0, 0, false>(ctx, s, args, nn, layout, st);
+        return any ? launch_single<1, 1, false>(ctx, s, args, nn, layout, st)
+                   : launch_single<1, 0, false>(ctx, s, args, nn, layout, st);
     }
     if (!refmt)
-        return any ? launch_single<0, 1, true>(ctx, s, args, nn, wide, st)
-                   : launch_single<0, 0, true>(ctx, s, args, nn, wide, st);
-    return any ? launch_single<1, 1, true>(ctx, s, args, nn, wide, st)
-               : launch_single<1, 0, true>(ctx, s, args, nn, wide, st);
+        return any ? launch_single<0, 1, true>(ctx, s, args, nn, layout, st)
+                   : launch_single<0, 0, true>(ctx, s, args, nn, layout, st);
+    return any ? launch_single<1, 1, true>(ctx, s, args, nn, layout, st)
+               : launch_single<1, 0, true>(ctx, s, args, nn, layout, st);
 }
 
 // Up to kMaxBatchItems ray sets (any scenes, same leaf test and layout) in one
@@ -630,15 +668,17 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
         const Scene* s = find_scene(ctx, it.scene_id);
         if (!s) return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: unknown scene %u", i, it.scene_id);
         if (!s->built) return fail(ctx, RTQ_ENOTBUILT, "trace_batch: item %u: scene %u has no BVH", i, it.scene_id);
-        if (it.flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT | RTQ_TRACE_BINARY))
+        if (it.flags & ~(RTQ_TRACE_ANY | RTQ_TRACE_LEAF_REFMT | RTQ_TRACE_BINARY | RTQ_TRACE_LEARNED))
             return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: unknown flags 0x%x", i, it.flags);
         if ((it.flags & RTQ_TRACE_LEAF_REFMT) != leaf_flags)
             return fail(ctx, RTQ_EINVAL, "trace_batch: all items must use the same leaf test");
         if (it.n_rays && (!it.rays_dev || !it.hits_dev))
             return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: null buffer", i);
-        const bool wide = s->wide != nullptr && !(it.flags & RTQ_TRACE_BINARY) && ctx->prefer_wide;
-        if (layout < 0) layout = wide;
-        if (layout != (int)wide) return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
+        const int lay = pick_layout(ctx, s, it.flags);
+        if (lay < 0) return RTQ_ENOTBUILT;
+        const bool wide = lay == kWide;
+        if (layout < 0) layout = lay;
+        if (layout != lay) return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
         BatchItem& b = table.item[i];
         b.nodes = wide ? s->wide : s->nodes;
         b.tris = s->tris;
@@ -657,11 +697,15 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
     args.n_items = n_items;
     TraceScratch& sc = scratch ? *scratch : ctx->scratch;
     const uint32_t nt = (uint32_t)total;
-    if (leaf_flags)
-        return layout ? launch<1, 2, false, true, true>(ctx, sc, depth, args, table, nt, size_only, st)
-                      : launch<1, 2, false, false, true>(ctx, sc, depth, args, table, nt, size_only, st);
-    return layout ? launch<0, 2, false, true, true>(ctx, sc, depth, args, table, nt, size_only, st)
-                  : launch<0, 2, false, false, true>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (leaf_flags) {
+        if (layout == kWide) return launch<1, 2, false, kWide, true>(ctx, sc, depth, args, table, nt, size_only, st);
+        if (layout == kOrdered)
+            return launch<1, 2, false, kOrdered, true>(ctx, sc, depth, args, table, nt, size_only, st);
+        return launch<1, 2, false, kBinary, true>(ctx, sc, depth, args, table, nt, size_only, st);
+    }
+    if (layout == kWide) return launch<0, 2, false, kWide, true>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (layout == kOrdered) return launch<0, 2, false, kOrdered, true>(ctx, sc, depth, args, table, nt, size_only, st);
+    return launch<0, 2, false, kBinary, true>(ctx, sc, depth, args, table, nt, size_only, st);
 }
 
 }  // namespace rtq

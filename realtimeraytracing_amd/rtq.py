@@ -38,6 +38,8 @@ RTQ_TRACE_CLOSEST = 0x0
 RTQ_TRACE_ANY = 0x1
 RTQ_TRACE_LEAF_REFMT = 0x2
 RTQ_TRACE_BINARY = 0x4
+RTQ_TRACE_LEARNED = 0x8
+ORDER_INPUTS, ORDER_HIDDEN, ORDER_PARAMS, ORDER_SAMPLE = 32, 64, 6337, 34
 
 EXPORTED = [
     "rtq_abi_version", "rtq_status_string", "rtq_create", "rtq_destroy", "rtq_last_error",
@@ -47,6 +49,8 @@ EXPORTED = [
     "rtq_trace_host", "rtq_raygen_pinhole", "rtq_raygen_diffuse", "rtq_raygen_shadow",
     "rtq_trace_batch", "rtq_batch_create", "rtq_batch_launch", "rtq_batch_destroy",
     "rtq_launch_params_get", "rtq_launch_params_set", "rtq_autotune", "rtq_tune_save", "rtq_tune_load",
+    "rtq_bvh_export_trav", "rtq_order_log", "rtq_order_init", "rtq_order_grad", "rtq_order_train",
+    "rtq_order_predict", "rtq_order_apply", "rtq_order_save", "rtq_order_load",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -78,6 +82,12 @@ class BvhInfo(ctypes.Structure):
                 ("box_pad", ctypes.c_float), ("build_ms", ctypes.c_float),
                 ("node_width", ctypes.c_uint32), ("n_wide_nodes", ctypes.c_uint32),
                 ("wide_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class TrainParams(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("reserved", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64)]
 
 
 class Pinhole(ctypes.Structure):
@@ -137,6 +147,15 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_autotune": (st, [vp, ctypes.POINTER(BatchItem), u32, u32, ctypes.POINTER(TuneResult)]),
         "rtq_tune_save": (st, [vp, ctypes.c_char_p]),
         "rtq_tune_load": (st, [vp, ctypes.c_char_p]),
+        "rtq_bvh_export_trav": (st, [vp, u32, vp, vp, u64, ctypes.POINTER(u64)]),
+        "rtq_order_log": (st, [vp, u32, vp, u64, u32, u32, vp, u64, ctypes.POINTER(u64)]),
+        "rtq_order_init": (st, [vp, u64, vp]),
+        "rtq_order_grad": (st, [vp, vp, vp, u32, vp, ctypes.POINTER(ctypes.c_float)]),
+        "rtq_order_train": (st, [vp, vp, u64, ctypes.POINTER(TrainParams), vp, vp, vp, ctypes.POINTER(u32), vp]),
+        "rtq_order_predict": (st, [vp, vp, vp, u64, u32, vp]),
+        "rtq_order_apply": (st, [vp, u32, vp, ctypes.c_float, ctypes.c_float]),
+        "rtq_order_save": (st, [vp, ctypes.c_char_p, vp, vp, vp, u32]),
+        "rtq_order_load": (st, [vp, ctypes.c_char_p, vp, vp, vp, ctypes.POINTER(u32)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -234,6 +253,35 @@ class Context:
     def tune_save(self, path) -> None:
         self.check(self.lib.rtq_tune_save(self.h, str(path).encode()))
 
+    def order_init(self, seed: int, params_ptr: int) -> None:
+        self.check(self.lib.rtq_order_init(self.h, seed, params_ptr))
+
+    def order_grad(self, params_ptr: int, samples_ptr: int, n: int, grad_ptr: int) -> float:
+        loss = ctypes.c_float()
+        self.check(self.lib.rtq_order_grad(self.h, params_ptr, samples_ptr, n, grad_ptr, ctypes.byref(loss)))
+        return loss.value
+
+    def order_train(self, samples_ptr: int, n: int, tp: "TrainParams", params_ptr: int, m_ptr: int, v_ptr: int,
+                    step: int = 0) -> tuple[int, np.ndarray]:
+        """Returns (step after training, mean loss of each epoch)."""
+        st = ctypes.c_uint32(step)
+        losses = np.zeros(max(tp.epochs, 1), np.float32)
+        self.check(self.lib.rtq_order_train(self.h, samples_ptr, n, ctypes.byref(tp), params_ptr, m_ptr, v_ptr,
+                                            ctypes.byref(st), _ptr(losses)))
+        return st.value, losses[:tp.epochs]
+
+    def order_predict(self, params_ptr: int, features_ptr: int, n: int, stride: int, prob_ptr: int) -> None:
+        self.check(self.lib.rtq_order_predict(self.h, params_ptr, features_ptr, n, stride, prob_ptr))
+
+    def order_save(self, path, params_ptr: int, m_ptr: int, v_ptr: int, step: int) -> None:
+        self.check(self.lib.rtq_order_save(self.h, str(path).encode(), params_ptr, m_ptr, v_ptr, step))
+
+    def order_load(self, path, params_ptr: int, m_ptr: int | None = None, v_ptr: int | None = None) -> int:
+        st = ctypes.c_uint32()
+        self.check(self.lib.rtq_order_load(self.h, str(path).encode(), params_ptr, m_ptr, v_ptr,
+                                           ctypes.byref(st)))
+        return st.value
+
     def tune_load(self, path) -> None:
         self.check(self.lib.rtq_tune_load(self.h, str(path).encode()))
 
@@ -291,6 +339,32 @@ class Scene:
                                        ("boxes", "triangle_id", "left", "right", "parent",
                                         "sorted_indices", "sorted_codes"))))
         return out
+
+    def export_trav(self) -> tuple[np.ndarray, np.ndarray]:
+        """Binary traversal nodes (n x 16 float32; words 12-14 hold u32 refs and the
+        order word) and their child leaf ranges + depth (n x 4 uint32)."""
+        n = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_trav(self.ctx.h, self.id, None, None, 0, ctypes.byref(n)))
+        nodes = np.zeros((n.value, 16), np.float32)
+        ranges = np.zeros((n.value, 4), np.uint32)
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_trav(self.ctx.h, self.id, _ptr(nodes), _ptr(ranges), n.value,
+                                                        ctypes.byref(n)))
+        return nodes, ranges
+
+    def order_log(self, rays_ptr: int, n: int, samples_ptr: int, capacity: int, flags: int = 0,
+                  stride: int = 1) -> int:
+        """Traversal-log samples (ORDER_SAMPLE floats each) written to samples_ptr; returns their count."""
+        out = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.rtq_order_log(self.ctx.h, self.id, rays_ptr, n, flags, stride, samples_ptr,
+                                                  capacity, ctypes.byref(out)))
+        return out.value
+
+    def order_apply(self, params_ptr: int | None, confidence_any: float = 0.5,
+                    confidence_closest: float = 0.5) -> None:
+        """Order words from the model; a confidence >= 1 turns that mode's decisions off."""
+        self.ctx.check(self.ctx.lib.rtq_order_apply(self.ctx.h, self.id, params_ptr,
+                                                    ctypes.c_float(confidence_closest),
+                                                    ctypes.c_float(confidence_any)))
 
     def morton_codes(self, flags: int = 0) -> np.ndarray:
         out = np.zeros(self.n_tris, np.uint32)

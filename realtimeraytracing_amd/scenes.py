@@ -247,3 +247,9 @@ def soup_scene(n: int = 65536, seed: int = 0):
 
 def terrain_scene(res: int = 1024, seed: int = 0):
     return terrain(res, seed), identity_models()
+
+
+def occluded_scene(terrain_res: int = 1024, soup_n: int = 2_097_152, seed: int = 0):
+    """Terrain plus a random soup over the unit cube (soup seed = seed + 1): the
+    learned-ordering workload, where shadow rays are often occluded."""
+    return np.concatenate([terrain(terrain_res, seed), soup(soup_n, seed + 1)]), identity_models()

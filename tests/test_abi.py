@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from realtimeraytracing_amd import (AABB_DTYPE, EXPORTED, HIT_DTYPE, MODEL_DTYPE, NODE_DTYPE, RAY_DTYPE,
-                                    TRIANGLE_DTYPE, BuildParams, BvhInfo, Pinhole, load_library)
+                                    TRIANGLE_DTYPE, BatchItem, BuildParams, BvhInfo, LaunchParams, Pinhole,
+                                    TrainParams, TuneResult, load_library)
 from realtimeraytracing_amd.rtq import RTQ_EINVAL
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -52,9 +53,10 @@ SIZES_C = r"""
 #include <stdio.h>
 #include "rtq/rtq.h"
 int main(void) {
-    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rtq_triangle), sizeof(rtq_mesh_model),
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rtq_triangle), sizeof(rtq_mesh_model),
            sizeof(rtq_aabb), sizeof(rtq_bvh_node), sizeof(rtq_ray), sizeof(rtq_hit),
-           sizeof(rtq_build_params), sizeof(rtq_bvh_info), sizeof(rtq_pinhole));
+           sizeof(rtq_build_params), sizeof(rtq_bvh_info), sizeof(rtq_pinhole), sizeof(rtq_batch_item),
+           sizeof(rtq_launch_params), sizeof(rtq_tune_result), sizeof(rtq_train_params));
     return 0;
 }
 """
@@ -70,8 +72,8 @@ def test_headers_compile_and_sizes(tmp_path, compiler, std):
     subprocess.run([compiler, std, "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
                    check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
-    assert sizes == [64, 68, 32, 48, 32, 16, ctypes.sizeof(BuildParams), ctypes.sizeof(BvhInfo),
-                     ctypes.sizeof(Pinhole)]
+    assert sizes == [64, 68, 32, 48, 32, 16] + [ctypes.sizeof(c) for c in (
+        BuildParams, BvhInfo, Pinhole, BatchItem, LaunchParams, TuneResult, TrainParams)]
 
 
 def test_argument_errors_without_device():

@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--forms", default="plain,compressed,batched")
+    ap.add_argument("--max-leaf", type=int, default=4, help="triangles per traversal leaf (1..8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -93,7 +94,7 @@ def main() -> int:
     tris, models = scenes.terrain_scene(1024, seed=0)
     ctx = Context(0)
     sc = ctx.upload(tris, models)
-    info = sc.build(16, 4, 0, node_width=8)     # builds the binary and the 8-wide layouts
+    info = sc.build(16, args.max_leaf, 0, node_width=8)     # builds the binary and the 8-wide layouts
     cam = scenes.pinhole(**scenes.TERRAIN_CAMERA)
     n = cam.width * cam.height
     d_prim, d_phit = devmem.empty(n, RAY_DTYPE), devmem.empty(n, HIT_DTYPE)
@@ -257,7 +258,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "terrain 1024x1024 (2,093,058 tris), 1920x1080 primary hits -> "
                                "1 diffuse (closest) + 4 shadow (any) rays each",
-                   "form": best, "rays_per_step": live_rays, "max_leaf_size": 4, "search_radius": 16},
+                   "form": best, "rays_per_step": live_rays, "max_leaf_size": args.max_leaf, "search_radius": 16},
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
         "detail": {"forms": {f: rounded({k: v for k, v in r.items() if k != "roofline"}) | {

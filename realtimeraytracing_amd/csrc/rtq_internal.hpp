@@ -70,9 +70,10 @@ struct Scene {
 }  // namespace rtq
 
 namespace rtq {
-// Device scratch of persistent trace launches: counter[0] = next ray,
-// counter[1] = blocks done (both zero between launches: the last block of a
-// launch resets them), spill = the deep part of the traversal stacks.
+// Device scratch of persistent trace launches: counter = the per-part work
+// counters and the finished-block counter (all zero between launches: the
+// last block of a launch resets them), spill = the deep part of the traversal
+// stacks.
 struct TraceScratch {
     uint32_t* counter = nullptr;
     uint32_t* spill = nullptr;
@@ -89,7 +90,7 @@ struct rtq_ctx {
     uint32_t next_scene = 1;
     rtq::TraceScratch scratch;
     // traversal launch settings: refill a wave once `refill` lanes are idle,
-    // taking `chunk` consecutive rays per atomic on the global counter; trace
+    // taking `chunk` consecutive rays per atomic on a work counter; trace
     // 8-wide nodes when a scene has them and prefer_wide is set
     uint32_t refill = 16;
     uint32_t chunk = 128;

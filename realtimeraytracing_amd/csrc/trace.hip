@@ -2,10 +2,11 @@
 //
 // Replaces getClosestHitBVH / intersectBVH / rayTriangleIntersection
 // (raytracer.glsl:102-147, 182-295).  Design:
-//  - persistent waves: a wave refills its idle lanes from a global ray counter
-//    (one atomic per refill, lanes ranked with mbcnt over the ballot) whenever
-//    at least `refill` lanes are idle, so finished lanes do not wait for the
-//    slowest ray of the wave;
+//  - persistent waves: a wave refills its idle lanes from a pool of
+//    consecutive rays taken from a work counter (one atomic per chunk, lanes
+//    ranked with mbcnt over the ballot) whenever at least `refill` lanes are
+//    idle, so finished lanes do not wait for the slowest ray of the wave; the
+//    ray range is split into parts owned by XCDs, drained by their owner first;
 //  - two-child 64-B nodes (both child boxes in one 64-B fetch), nearer child
 //    first, farther child pushed; traversal stack of child refs in LDS for the
 //    top kStackLds entries ([slot][lane] so lanes hit distinct banks), deeper
@@ -26,7 +27,33 @@ namespace rtq {
 namespace {
 
 constexpr int kTraceThreads = 256;
-constexpr uint32_t kStackLds = 16;
+// Work distribution: every ray set (item) of a launch is split into kXcds
+// contiguous parts and part j of each item belongs to XCD j.  Workgroups are
+// dispatched to the XCDs round-robin, so a wave of block b drains XCD b %
+// kXcds's part of every item first (its rays then share nodes in that XCD's
+// L2), then steals parts of the other XCDs.  Part (item i, j) has counter
+// p = i * kXcds + j, 64 B apart; counter[kDry..] is a bit mask of exhausted
+// parts; counter[kArrive] counts finished blocks.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kMaxBatchItems = 64;
+constexpr uint32_t kMaxParts = kMaxBatchItems * kXcds;
+constexpr uint32_t kPartStride = 16;
+constexpr uint32_t kDry = kMaxParts * kPartStride;
+constexpr uint32_t kArrive = kDry + kMaxParts / 32;
+constexpr uint32_t kCounterWords = kArrive + kPartStride;
+
+// Traversal stack entries per lane held in LDS (deeper entries spill to HBM).
+#ifndef RTQ_STACK_LDS
+#define RTQ_STACK_LDS 16
+#endif
+constexpr uint32_t kStackLds = RTQ_STACK_LDS;
+// Optional VGPR budget of the trace kernels; occupancy is 512 / budget waves
+// per SIMD.
+#ifdef RTQ_TRACE_VGPRS
+#define RTQ_TRACE_OCCUPANCY __attribute__((amdgpu_num_vgpr(RTQ_TRACE_VGPRS)))
+#else
+#define RTQ_TRACE_OCCUPANCY
+#endif
 
 // One ray set of a batched launch: rays [begin, begin + n) of the launch's
 // global index space trace against one scene.
@@ -41,7 +68,6 @@ struct BatchItem {
     uint32_t pad;
 };
 static_assert(sizeof(BatchItem) == 48, "batch descriptor is 48 B");
-constexpr uint32_t kMaxBatchItems = 64;
 
 // Item descriptors of a batched launch, passed by value (3 KB of kernel
 // arguments) so a captured graph keeps them; empty for single-scene launches.
@@ -191,7 +217,7 @@ enum Layout : int {
 // MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: rays come from
 // several items (scenes), each lane takes its scene and buffers from its item.
 template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
-__global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<MULTI> table) {
+__global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(TraceArgs a, ItemTable<MULTI> table) {
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
     const uint32_t tid = threadIdx.x;
@@ -213,8 +239,14 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
     uint32_t order_bit = 0;
 
     int rid = -1;
-    // wave-uniform ray pool [pool_next, pool_end); dry once the counter passed n
-    uint32_t pool_next = 0, pool_end = 0;
+    // wave-uniform ray pool [pool_next, pool_end) of item pool_item, taken
+    // from the seq-th part of the wave's order; dry once every part is
+    // exhausted
+    uint32_t pool_next = 0, pool_end = 0, pool_item = 0;
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t n_items = MULTI ? a.n_items : 1u;
+    const uint32_t n_parts = n_items * kXcds;
+    uint32_t seq = 0;
     bool dry = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
     float ix = 0, iy = 0, iz = 0, oix = 0, oiy = 0, oiz = 0;
@@ -367,20 +399,63 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
 
     for (;;) {
         // ---- refill idle lanes from the wave's pool of consecutive rays; the
-        // pool is topped up with one atomic on the global counter per chunk
+        // pool is topped up with one atomic on a part counter per chunk
         const bool need = (rid < 0) && !dry;
         const uint64_t need_mask = __ballot(need);
         if (need_mask) {
             if (pool_next == pool_end) {
                 const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need_mask) - 1u;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(a.counter, a.chunk);
-                base = __shfl(base, (int)leader);
-                if (base >= a.n) {
+                uint32_t lo = 0, hi = 0, item = 0;
+                if (lane == leader) {
+                    // the current part, then the next ones of the wave's order
+                    // (own XCD's part of each item, then the next XCD's);
+                    // parts marked exhausted are skipped without an atomic
+                    const unsigned long long* dry_mask =
+                        reinterpret_cast<const unsigned long long*>(a.counter + kDry);
+                    unsigned long long gone = 0;
+                    uint32_t gone_word = kNone;
+                    for (const uint32_t first = seq; seq < n_parts; ++seq) {
+                        item = seq % n_items;
+                        const uint32_t j = (xcd + seq / n_items) % kXcds;
+                        const uint32_t p = item * kXcds + j;
+                        // up to 16 parts are probed with their own atomics; more
+                        // (large batches) consult the exhausted-part mask so a
+                        // wave's end-of-launch probing stays short
+                        if (n_parts > 16 && seq != first) {
+                            if (p / 64 != gone_word) {
+                                gone_word = p / 64;
+                                gone = __hip_atomic_load(dry_mask + gone_word, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                            }
+                            if ((gone >> (p % 64)) & 1ull) continue;
+                        }
+                        uint32_t ib = 0, in = a.n;
+                        if constexpr (MULTI) {
+                            ib = s_items[item].begin;
+                            in = s_items[item].n;
+                        }
+                        const uint32_t pb = ib + (uint32_t)(((uint64_t)in * j) / kXcds);
+                        const uint32_t pe = ib + (uint32_t)(((uint64_t)in * (j + 1)) / kXcds);
+                        const uint32_t base = atomicAdd(a.counter + p * kPartStride, a.chunk);
+                        if (base < pe - pb) {
+                            lo = pb + base;
+                            hi = min(lo + a.chunk, pe);
+                            break;
+                        }
+                        if (n_parts > 16)
+                            atomicOr(const_cast<unsigned long long*>(dry_mask) + p / 64, 1ull << (p % 64));
+                    }
+                }
+                lo = __shfl(lo, (int)leader);
+                hi = __shfl(hi, (int)leader);
+                item = __shfl(item, (int)leader);
+                seq = __shfl(seq, (int)leader);
+                if (lo == hi) {
                     dry = true;
                 } else {
-                    pool_next = base;
-                    pool_end = min(base + a.chunk, a.n);
+                    pool_next = lo;
+                    pool_end = hi;
+                    pool_item = item;
                 }
             }
             if (!dry) {
@@ -394,13 +469,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
                     const rtq_ray* rays_p = a.rays;
                     local = r;
                     if constexpr (MULTI) {
-                        // the item holding global ray r: last one with begin <= r
-                        uint32_t lo = 0, hi = a.n_items - 1;
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi + 1) >> 1;
-                            if (s_items[mid].begin <= r) lo = mid; else hi = mid - 1;
-                        }
-                        const BatchItem& it = s_items[lo];
+                        const BatchItem& it = s_items[pool_item];
                         nodes_p = it.nodes;
                         tris_p = it.tris;
                         rays_p = it.rays;
@@ -519,17 +588,21 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(TraceArgs a, ItemTable<
             if (active == 0 || (!dry && (uint32_t)__popcll(idle) >= a.refill)) break;
         }
     }
-    // A wave leaves the loop only after its last fetch ran past n, so when every
-    // block has arrived here no fetch is outstanding: the last block resets the
-    // ray counter (counter[0]) and the arrival counter (counter[1]) for the next
-    // launch on the stream.
+    // A wave leaves the loop only after every part ran dry for it, so when
+    // every block has arrived here no fetch is outstanding: the last block
+    // resets the part counters, the exhausted-part mask and the arrival counter
+    // for the next launch on the stream.
+    __shared__ uint32_t s_last;
     __syncthreads();
     if (tid == 0) {
         __threadfence();
-        if (atomicAdd(a.counter + 1, 1u) == gridDim.x - 1) {
-            atomicExch(a.counter, 0u);
-            atomicExch(a.counter + 1, 0u);
-        }
+        s_last = atomicAdd(a.counter + kArrive, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (uint32_t p = tid; p < n_parts; p += kTraceThreads) atomicExch(a.counter + p * kPartStride, 0u);
+        if (tid < (n_parts + 31) / 32) atomicExch(a.counter + kDry + tid, 0u);
+        if (tid == 0) atomicExch(a.counter + kArrive, 0u);
     }
 }
 
@@ -609,8 +682,8 @@ int pick_layout(rtq_ctx* ctx, const Scene* s, uint32_t flags) {
 }  // namespace
 
 rtq_status scratch_alloc(rtq_ctx* ctx, TraceScratch* sc) {
-    RTQ_HIP(ctx, dev_alloc(&sc->counter, 64));
-    RTQ_HIP(ctx, hipMemset(sc->counter, 0, 64 * sizeof(uint32_t)));
+    RTQ_HIP(ctx, dev_alloc(&sc->counter, kCounterWords));
+    RTQ_HIP(ctx, hipMemset(sc->counter, 0, kCounterWords * sizeof(uint32_t)));
     return RTQ_OK;
 }
 
@@ -652,8 +725,9 @@ rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, 
 }
 
 // Up to kMaxBatchItems ray sets (any scenes, same leaf test and layout) in one
-// persistent launch: waves take chunks from one counter over the concatenated
-// ray index space, so uneven sets balance across the whole chip.
+// persistent launch: waves take chunks of the concatenated ray index space
+// (their XCD's part first, then any other part), so uneven sets balance across
+// the whole chip.
 rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
                        bool size_only, hipStream_t st) {
     if (n_items == 0) return RTQ_OK;

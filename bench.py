@@ -11,18 +11,29 @@ resident in HBM.  Mrays/s counts only rays with a non-empty interval (every
 primary ray hits, so all 10,368,000 are live), M = 10^6.
 
 Forms measured on the same scene and rays (hit records are bit-identical
-between them, checked in tests/test_gpu_wide.py and tests/test_gpu_batch.py):
+between them; closest-hit records and any-hit occlusion flags are compared
+here too, `exact`):
   plain       binary 64-B nodes, one launch per ray set, default settings
   compressed  8-wide 128-B nodes with 8-bit outward-rounded child boxes
   batched     both ray sets in one persistent launch replayed from a
               hipGraph, settings (refill, chunk, node layout) from the
               autotuner
+  learned     the batched form with the learned child order (model trained
+              on this scene's own traversal logs before timing; any-hit
+              decisions only, closest-hit rays keep the nearer child)
 `value` is the fastest form; `detail.forms` has every form.
 
+`detail.occluded` measures the learned-ordering workload: the same terrain
+plus 2,097,152 random-soup triangles over the unit cube, 2048 x 2048 primary
+hits -> 4M diffuse + 16M shadow rays (nearly all occluded): plain, learned
+order (two launches) and learned order batched in one graph-replayed launch,
+with the same exactness check.
+
 Prints one JSON line with the metric, a roofline object for the dominant
-trace kernel of that form (algorithmic bytes per launch from the traversal
-counters over its HIP-event duration) and the CPU baseline (the reference
-traversal restated in C, run on this host's cores over the same rays).
+trace kernel of the best form (algorithmic bytes per launch from the
+traversal counters over its HIP-event duration) and the CPU baseline (the
+reference traversal restated in C, run on this host's cores over the same
+rays).
 """
 from __future__ import annotations
 
@@ -49,9 +60,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--forms", default="plain,compressed,batched")
+    ap.add_argument("--forms", default="plain,compressed,batched,learned")
     ap.add_argument("--max-leaf", type=int, default=4, help="triangles per traversal leaf (1..8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-occluded", action="store_true", help="skip the terrain + soup learned-order workload")
+    ap.add_argument("--occluded-res", type=int, default=2048, help="primary image side of the occluded workload")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
@@ -71,6 +84,90 @@ def read_traffic(path: str, kernel: str):
     return None
 
 
+class Workload:
+    """A scene with primary hits and the diffuse + shadow rays built from them."""
+
+    def __init__(self, ctx, tris, models, cam, stream, max_leaf, node_width):
+        from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_BINARY, devmem, scenes
+        import torch
+        self.ctx, self.sp = ctx, stream
+        self.sc = ctx.upload(tris, models)
+        self.info = self.sc.build(16, max_leaf, 0, node_width=node_width)
+        n = cam.width * cam.height
+        self.prim, self.phit = devmem.empty(n, RAY_DTYPE), devmem.empty(n, HIT_DTYPE)
+        ctx.raygen_pinhole(cam, self.prim.data_ptr(), stream)
+        self.sc.trace(self.prim.data_ptr(), n, self.phit.data_ptr(), RTQ_TRACE_BINARY, stream)
+        self.diff, self.shad = devmem.empty(n, RAY_DTYPE), devmem.empty(4 * n, RAY_DTYPE)
+        self.sc.raygen_diffuse(self.prim.data_ptr(), self.phit.data_ptr(), n, scenes.DIFFUSE_SEED,
+                               scenes.SECONDARY_EPS, self.diff.data_ptr(), stream)
+        self.sc.raygen_shadow(self.prim.data_ptr(), self.phit.data_ptr(), n, scenes.LIGHTS, scenes.SECONDARY_EPS,
+                              self.shad.data_ptr(), stream)
+        self.hd, self.hs = devmem.empty(n, HIT_DTYPE), devmem.empty(4 * n, HIT_DTYPE)
+        torch.cuda.synchronize()
+        phit = devmem.to_host(self.phit, HIT_DTYPE)
+        self.n_diff, self.n_shad = n, 4 * n
+        n_live = int((phit["id"] >= 0).sum())
+        self.live_rays = 5 * n_live
+
+    def sets(self):
+        from realtimeraytracing_amd import RTQ_TRACE_ANY
+        return (("diffuse", self.diff, self.n_diff, self.hd, 0),
+                ("shadow", self.shad, self.n_shad, self.hs, RTQ_TRACE_ANY))
+
+    def counters(self, layout_flags, node_bytes):
+        """Per-set traversal counters of a layout (outside any timed region)."""
+        from realtimeraytracing_amd import devmem
+        import torch
+        out = {}
+        for name, d_rays, cnt, d_hits, flags in self.sets():
+            dv, dt = devmem.empty(cnt, np.uint32), devmem.empty(cnt, np.uint32)
+            self.sc.trace_stats(d_rays.data_ptr(), cnt, d_hits.data_ptr(), dv.data_ptr(), dt.data_ptr(),
+                                flags | layout_flags, self.sp)
+            torch.cuda.synchronize()
+            v = int(devmem.to_host(dv, np.uint32).astype(np.int64).sum())
+            t = int(devmem.to_host(dt, np.uint32).astype(np.int64).sum())
+            out[name] = dict(rays=cnt, node_visits=v, tri_tests=t,
+                             bytes=cnt * (RAY_BYTES + HIT_BYTES) + v * node_bytes + t * TRI_BYTES)
+        return out
+
+    def hits(self):
+        from realtimeraytracing_amd import HIT_DTYPE, devmem
+        import torch
+        torch.cuda.synchronize()
+        return devmem.to_host(self.hd, HIT_DTYPE), devmem.to_host(self.hs, HIT_DTYPE)
+
+    def release(self):
+        self.sc.release()
+
+
+def same_results(ref, got) -> bool:
+    """Closest-hit records bit-identical, any-hit occlusion flags identical."""
+    (rd, rs), (gd, gs) = ref, got
+    return bool(np.array_equal(np.ascontiguousarray(rd).view(np.uint8), np.ascontiguousarray(gd).view(np.uint8))
+                and np.array_equal(rs["id"] >= 0, gs["id"] >= 0))
+
+
+def train_order(ctx, wl, stream):
+    """Order model from this scene's own traversal logs (about 200k shadow and
+    100k diffuse rays sampled), applied to the scene's nodes; untimed setup."""
+    import torch
+    from realtimeraytracing_amd.order import OrderModel
+    t0 = time.perf_counter()
+    model = OrderModel(ctx, seed=1)
+    torch.cuda.synchronize()
+    ss, ks = model.log(wl.sc, wl.shad, wl.n_shad, True, stride=max(1, wl.n_shad // 200_000), capacity=1 << 22)
+    sd, kd = model.log(wl.sc, wl.diff, wl.n_diff, False, stride=max(1, wl.n_diff // 100_000), capacity=1 << 22)
+    samples = torch.cat([ss, sd]).contiguous()
+    losses = model.train(samples, epochs=6, batch=1024, lr=2e-3) if len(samples) else []
+    # decisions only for any-hit rays, and only when the logs held any-hit samples
+    model.apply(wl.sc, confidence_any=0.5 if ks else 1.0, confidence_closest=1.0)
+    torch.cuda.synchronize()
+    return {"samples_any": ks, "samples_closest": kd, "epochs": 6,
+            "loss_first": round(float(losses[0]), 4) if len(losses) else None,
+            "loss_last": round(float(losses[-1]), 4) if len(losses) else None,
+            "setup_s": round(time.perf_counter() - t0, 2)}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -78,7 +175,7 @@ def main() -> int:
                                    "the trace path shards by ray ranges with a replicated BVH (see DESIGN.md)"}))
         return 2
     import torch
-    from realtimeraytracing_amd import (HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_ANY, RTQ_TRACE_BINARY, Context, devmem,
+    from realtimeraytracing_amd import (RTQ_TRACE_ANY, RTQ_TRACE_BINARY, RTQ_TRACE_LEARNED, Context, devmem,
                                         scenes)
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -89,47 +186,6 @@ def main() -> int:
     torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
     assert sp != 0
-
-    t0 = time.time()
-    tris, models = scenes.terrain_scene(1024, seed=0)
-    ctx = Context(0)
-    sc = ctx.upload(tris, models)
-    info = sc.build(16, args.max_leaf, 0, node_width=8)     # builds the binary and the 8-wide layouts
-    cam = scenes.pinhole(**scenes.TERRAIN_CAMERA)
-    n = cam.width * cam.height
-    d_prim, d_phit = devmem.empty(n, RAY_DTYPE), devmem.empty(n, HIT_DTYPE)
-    ctx.raygen_pinhole(cam, d_prim.data_ptr(), sp)
-    sc.trace(d_prim.data_ptr(), n, d_phit.data_ptr(), RTQ_TRACE_BINARY, sp)
-    d_diff, d_shad = devmem.empty(n, RAY_DTYPE), devmem.empty(4 * n, RAY_DTYPE)
-    sc.raygen_diffuse(d_prim.data_ptr(), d_phit.data_ptr(), n, scenes.DIFFUSE_SEED, scenes.SECONDARY_EPS,
-                      d_diff.data_ptr(), sp)
-    sc.raygen_shadow(d_prim.data_ptr(), d_phit.data_ptr(), n, scenes.LIGHTS, scenes.SECONDARY_EPS,
-                     d_shad.data_ptr(), sp)
-    d_hd, d_hs = devmem.empty(n, HIT_DTYPE), devmem.empty(4 * n, HIT_DTYPE)
-    torch.cuda.synchronize()
-    phit = devmem.to_host(d_phit, HIT_DTYPE)
-    n_live = int((phit["id"] >= 0).sum())
-    n_diff, n_shad = n, 4 * n
-    live_rays = n_live + 4 * n_live
-    setup_s = time.time() - t0
-
-    # traversal counters of each layout (outside the timed region): algorithmic bytes per launch
-    def layout_counters(layout, node_bytes):
-        counters = {}
-        for name, d_rays, cnt, d_hits, flags in (("diffuse", d_diff, n_diff, d_hd, 0),
-                                                  ("shadow", d_shad, n_shad, d_hs, RTQ_TRACE_ANY)):
-            dv, dt = devmem.empty(cnt, np.uint32), devmem.empty(cnt, np.uint32)
-            sc.trace_stats(d_rays.data_ptr(), cnt, d_hits.data_ptr(), dv.data_ptr(), dt.data_ptr(),
-                           flags | layout, sp)
-            torch.cuda.synchronize()
-            v = int(devmem.to_host(dv, np.uint32).astype(np.int64).sum())
-            t = int(devmem.to_host(dt, np.uint32).astype(np.int64).sum())
-            counters[name] = dict(rays=cnt, node_visits=v, tri_tests=t,
-                                  bytes=cnt * (RAY_BYTES + HIT_BYTES) + v * node_bytes + t * TRI_BYTES)
-        return counters
-
-    default_params = ctx.launch_params()
-    counters_by_layout = {"binary": layout_counters(RTQ_TRACE_BINARY, 64), "wide": layout_counters(0, 128)}
 
     def time_steps(step):
         for _ in range(args.warmup):
@@ -144,25 +200,48 @@ def main() -> int:
         torch.cuda.synchronize()
         return start.elapsed_time(stop) / args.steps, evs
 
+    def two_launches(wl, layout):
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            wl.sc.trace(wl.diff.data_ptr(), wl.n_diff, wl.hd.data_ptr(), layout, sp)
+            if ev is not None:
+                ev[1].record(stream)
+            wl.sc.trace(wl.shad.data_ptr(), wl.n_shad, wl.hs.data_ptr(), RTQ_TRACE_ANY | layout, sp)
+            if ev is not None:
+                ev[2].record(stream)
+        return step
+
+    def graph_step(graph):
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            graph.launch(sp)
+            if ev is not None:
+                ev[1].record(stream)
+                ev[2].record(stream)
+        return step
+
+    t0 = time.time()
+    ctx = Context(0)
+    tris, models = scenes.terrain_scene(1024, seed=0)
+    cam = scenes.pinhole(**scenes.TERRAIN_CAMERA)
+    wl = Workload(ctx, tris, models, cam, sp, args.max_leaf, node_width=8)   # binary + 8-wide layouts
+    info = wl.info
+    setup_s = time.time() - t0
+
+    default_params = ctx.launch_params()
+    counters_by_layout = {"binary": wl.counters(RTQ_TRACE_BINARY, 64), "wide": wl.counters(0, 128)}
+    reference = None
     results = {}
+    learned_setup = None
     for form in args.forms.split(","):
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide))
         tuned = None
         if form in ("plain", "compressed"):
             layout = RTQ_TRACE_BINARY if form == "plain" else 0
             lname = "binary" if form == "plain" else "wide"
-
-            def step(ev=None):
-                if ev is not None:
-                    ev[0].record(stream)
-                sc.trace(d_diff.data_ptr(), n_diff, d_hd.data_ptr(), layout, sp)
-                if ev is not None:
-                    ev[1].record(stream)
-                sc.trace(d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY | layout, sp)
-                if ev is not None:
-                    ev[2].record(stream)
-
-            ms_per_step, evs = time_steps(step)
+            ms_per_step, evs = time_steps(two_launches(wl, layout))
             diff_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
             shad_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
             counters = counters_by_layout[lname]
@@ -170,26 +249,22 @@ def main() -> int:
             dom_ms = shad_ms if dom == "shadow" else diff_ms
             dom_bytes = counters[dom]["bytes"]
             kernel = f"k_trace<watertight,{'any' if dom == 'shadow' else 'closest'},{lname}>"
-        elif form == "batched":
+        elif form in ("batched", "learned"):
             # both ray sets in one persistent launch, replayed from a hipGraph,
             # launch settings (incl. node layout) from the autotuner
-            items = ctx.items([(sc, d_diff.data_ptr(), n_diff, d_hd.data_ptr(), 0),
-                               (sc, d_shad.data_ptr(), n_shad, d_hs.data_ptr(), RTQ_TRACE_ANY)])
+            extra = 0
+            if form == "learned":
+                learned_setup = train_order(ctx, wl, sp)
+                extra = RTQ_TRACE_LEARNED
+                counters_by_layout["learned"] = wl.counters(RTQ_TRACE_LEARNED, 64)
+            items = ctx.items([(wl.sc, wl.diff.data_ptr(), wl.n_diff, wl.hd.data_ptr(), extra),
+                               (wl.sc, wl.shad.data_ptr(), wl.n_shad, wl.hs.data_ptr(), RTQ_TRACE_ANY | extra)])
             tr = ctx.autotune(items, reps=2)
             tuned = {"refill": tr.best.refill, "chunk": tr.best.chunk, "prefer_wide": tr.best.prefer_wide,
                      "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4), "default_ms": round(tr.default_ms, 4)}
-            lname = "wide" if tr.best.prefer_wide else "binary"
+            lname = "learned" if form == "learned" else ("wide" if tr.best.prefer_wide else "binary")
             graph = ctx.batch(items)
-
-            def step(ev=None):
-                if ev is not None:
-                    ev[0].record(stream)
-                graph.launch(sp)
-                if ev is not None:
-                    ev[1].record(stream)
-                    ev[2].record(stream)
-
-            ms_per_step, evs = time_steps(step)
+            ms_per_step, evs = time_steps(graph_step(graph))
             graph.destroy()
             diff_ms = shad_ms = None
             counters = counters_by_layout[lname]
@@ -199,12 +274,15 @@ def main() -> int:
             kernel = f"k_trace<watertight,per-item,{lname},batch>"
         else:
             raise SystemExit(f"unknown form {form}")
+        got = wl.hits()
+        if reference is None:
+            reference = got
         achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-        node_bytes = 64 if lname == "binary" else 128
+        node_bytes = 128 if lname == "wide" else 64
         results[form] = {
-            "Mrays_s": live_rays / (ms_per_step * 1e-3) / 1e6,
+            "Mrays_s": wl.live_rays / (ms_per_step * 1e-3) / 1e6,
             "ms_per_step": ms_per_step, "diffuse_ms": diff_ms, "shadow_ms": shad_ms,
-            "layout": lname, "tuned": tuned, "counters": counters,
+            "layout": lname, "tuned": tuned, "counters": counters, "exact": same_results(reference, got),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.traffic_file, kernel),
                          "kernel": kernel, "launch_ms": round(dom_ms, 4),
@@ -221,10 +299,11 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
-        nodes = sc.export_ref()
+        from realtimeraytracing_amd import RAY_DTYPE
+        nodes = wl.sc.export_ref()
         v9 = oracle.world_vertices(tris, models)
-        diff = devmem.to_host(d_diff, RAY_DTYPE)
-        shad = devmem.to_host(d_shad, RAY_DTYPE)
+        diff = devmem.to_host(wl.diff, RAY_DTYPE)
+        shad = devmem.to_host(wl.shad, RAY_DTYPE)
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = int(os.environ.get("OMP_NUM_THREADS", cores))
         t1 = time.perf_counter()
@@ -235,10 +314,64 @@ def main() -> int:
         oracle.trace_bvh(nodes, v9, diff, threads=cores)
         oracle.trace_bvh(nodes, v9, shad, mode=oracle.MODE_ANY, threads=cores)
         cpu2_s = time.perf_counter() - t2
-        cpu = {"value": round(live_rays / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-               "sample": f"all {n_diff} diffuse + {n_shad} shadow rays of one step, reference traversal "
+        cpu = {"value": round(wl.live_rays / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+               "sample": f"all {wl.n_diff} diffuse + {wl.n_shad} shadow rays of one step, reference traversal "
                          f"semantics (RefMT, no pruning), {cpu_s:.2f} s",
-               "watertight_pruned_Mrays_s": round(live_rays / cpu2_s / 1e6, 3)}
+               "watertight_pruned_Mrays_s": round(wl.live_rays / cpu2_s / 1e6, 3)}
+    wl.release()
+
+    # learned-ordering workload: terrain + soup, plain against learned order
+    occluded = None
+    if not args.no_occluded:
+        t3 = time.time()
+        otris, omodels = scenes.occluded_scene(1024, 2_097_152, seed=0)
+        ocam = scenes.pinhole(**{**scenes.TERRAIN_CAMERA, "width": args.occluded_res,
+                                 "height": args.occluded_res})
+        owl = Workload(ctx, otris, omodels, ocam, sp, args.max_leaf, node_width=2)
+        n_otris = len(otris)
+        del otris
+        oforms = {}
+        oref = None
+        for form in ("plain", "learned", "learned_batched"):
+            flags = RTQ_TRACE_BINARY
+            setup = None
+            if form == "learned":
+                setup = train_order(ctx, owl, sp)
+            if form != "plain":
+                flags = RTQ_TRACE_LEARNED
+            if form == "learned_batched":
+                ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide))
+                items = ctx.items([(owl.sc, owl.diff.data_ptr(), owl.n_diff, owl.hd.data_ptr(), flags),
+                                   (owl.sc, owl.shad.data_ptr(), owl.n_shad, owl.hs.data_ptr(),
+                                    RTQ_TRACE_ANY | flags)])
+                ctx.autotune(items, reps=2)
+                graph = ctx.batch(items)
+                ms_per_step, evs = time_steps(graph_step(graph))
+                graph.destroy()
+            else:
+                ms_per_step, evs = time_steps(two_launches(owl, flags))
+            got = owl.hits()
+            if oref is None:
+                oref = got
+            c = owl.counters(flags, 64)
+            split = form != "learned_batched"
+            oforms[form] = {
+                "Mrays_s": round(owl.live_rays / (ms_per_step * 1e-3) / 1e6, 2),
+                "ms_per_step": round(ms_per_step, 4),
+                "diffuse_ms": round(float(np.mean([e[0].elapsed_time(e[1]) for e in evs])), 4) if split else None,
+                "shadow_ms": round(float(np.mean([e[1].elapsed_time(e[2]) for e in evs])), 4) if split else None,
+                "shadow_node_visits": c["shadow"]["node_visits"], "shadow_tri_tests": c["shadow"]["tri_tests"],
+                "occluded_fraction": round(float((got[1]["id"] >= 0).mean()), 4),
+                "exact": same_results(oref, got), "order_model": setup}
+        occluded = {
+            "workload": f"terrain 1024x1024 + 2,097,152 soup triangles ({n_otris:,} tris), "
+                        f"{args.occluded_res}x{args.occluded_res} primary hits -> 1 diffuse + 4 shadow rays each",
+            "rays_per_step": owl.live_rays, "forms": oforms,
+            "learned_vs_plain": round(oforms["learned"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
+            "learned_batched_vs_plain": round(oforms["learned_batched"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
+            "shadow_speedup": round(oforms["plain"]["shadow_ms"] / oforms["learned"]["shadow_ms"], 3),
+            "setup_s": round(time.time() - t3, 1)}
+        owl.release()
 
     def rounded(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
@@ -258,19 +391,23 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "terrain 1024x1024 (2,093,058 tris), 1920x1080 primary hits -> "
                                "1 diffuse (closest) + 4 shadow (any) rays each",
-                   "form": best, "rays_per_step": live_rays, "max_leaf_size": args.max_leaf, "search_radius": 16},
+                   "form": best, "rays_per_step": wl.live_rays, "max_leaf_size": args.max_leaf,
+                   "search_radius": 16},
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
         "detail": {"forms": {f: rounded({k: v for k, v in r.items() if k != "roofline"}) | {
                         "roofline_frac": r["roofline"]["frac"]} for f, r in results.items()},
                    "speedup_vs_plain": round(res["Mrays_s"] / results["plain"]["Mrays_s"], 3)
                    if "plain" in results else None,
+                   "learned_vs_plain": round(results["learned"]["Mrays_s"] / results["plain"]["Mrays_s"], 3)
+                   if "learned" in results and "plain" in results else None,
+                   "order_model": learned_setup,
+                   "occluded": occluded,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
                    "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
                    "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},
     }
     print(json.dumps(out))
-    sc.release()
     ctx.close()
     if args.copy_calibration_mb > 0:
         nbytes = args.copy_calibration_mb << 20

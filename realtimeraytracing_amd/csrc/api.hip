@@ -63,6 +63,8 @@ void free_build(Scene* s) {
     release(s->wide);
     s->built = false;
     s->ordered = false;
+    s->order_closest = false;
+    s->order_any = false;
     s->n_trav = 0;
     s->n_wide = 0;
     s->wide_depth = 0;

@@ -773,6 +773,8 @@ rtq_status rtq_order_apply(rtq_ctx* ctx, uint32_t scene_id, const float* params_
     RTQ_HIP(ctx, hipGetLastError());
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     s->ordered = params_dev != nullptr;
+    s->order_closest = s->ordered && confidence_closest < 1.0f;
+    s->order_any = s->ordered && confidence_any < 1.0f;
     return RTQ_OK;
 }
 

@@ -55,7 +55,9 @@ struct Scene {
     rtq_bvh_node* ref_nodes = nullptr;  // 2N - 1 pre-order nodes (reference layout)
     float4* nodes = nullptr;            // traversal nodes, 4 float4 (64 B) each
     uint4* node_aux = nullptr;          // per traversal node: child leaf ranges [A,B) [B,C), depth
-    bool ordered = false;               // learned child-order bits stored in the nodes
+    bool ordered = false;               // learned child-order words stored in the nodes
+    bool order_closest = false;         // ... with decisions for closest-hit rays
+    bool order_any = false;             // ... with decisions for any-hit rays
     float4* tris = nullptr;             // traversal triangles, leaf order, 3 float4 (48 B) each
     uint32_t n_trav = 0;
     float4* wide = nullptr;             // compressed 8-wide nodes, 8 float4 (128 B) each

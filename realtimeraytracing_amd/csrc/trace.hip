@@ -65,7 +65,7 @@ struct BatchItem {
     uint32_t begin;
     uint32_t n;
     uint32_t any;
-    uint32_t pad;
+    uint32_t learned;  // take the learned child order where the nodes hold it
 };
 static_assert(sizeof(BatchItem) == 48, "batch descriptor is 48 B");
 
@@ -235,8 +235,10 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
     rtq_hit* hits_p = a.hits;
     uint32_t local = 0;              // ray index within its ray set
     bool lane_any = MODE == 1;
-    // learned order: bit (octant + 8 * any) of a node's order word set = child 0 first
-    uint32_t order_bit = 0;
+    // learned order: bit (octant + 8 * any) of a node's order word set = child 0
+    // first, bit 16 + that = decision used; order_mask clears the word for
+    // lanes whose ray set does not use the learned order
+    uint32_t order_bit = 0, order_mask = 0xFFFFFFFFu;
 
     int rid = -1;
     // wave-uniform ray pool [pool_next, pool_end) of item pool_item, taken
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                 bool first0;
                 if constexpr (LAYOUT == kOrdered) {
                     // learned decision where its confidence bit is set, else the nearer child
-                    const uint32_t ow = __float_as_uint(n3.z) >> order_bit;
+                    const uint32_t ow = (__float_as_uint(n3.z) & order_mask) >> order_bit;
                     first0 = ((ow >> 16) & 1u) ? (ow & 1u) != 0 : tn0 <= tn1;
                 } else {
                     first0 = tn0 <= tn1;
@@ -476,6 +478,7 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                         hits_p = it.hits;
                         local = r - it.begin;
                         lane_any = it.any != 0;
+                        order_mask = it.learned ? 0xFFFFFFFFu : 0u;
                     }
                     const float4* rp = reinterpret_cast<const float4*>(rays_p + local);
                     const float4 r0 = rp[0], r1 = rp[1];
@@ -674,7 +677,9 @@ int pick_layout(rtq_ctx* ctx, const Scene* s, uint32_t flags) {
             fail(ctx, RTQ_ENOTBUILT, "trace: RTQ_TRACE_LEARNED needs order bits (rtq_order_apply)");
             return -1;
         }
-        return kOrdered;
+        // a mode without decisions takes the nearer child everywhere: the plain kernel
+        const bool decisions = (flags & RTQ_TRACE_ANY) ? s->order_any : s->order_closest;
+        return decisions ? kOrdered : kBinary;
     }
     return (s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide) ? kWide : kBinary;
 }
@@ -751,8 +756,11 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
         const int lay = pick_layout(ctx, s, it.flags);
         if (lay < 0) return RTQ_ENOTBUILT;
         const bool wide = lay == kWide;
-        if (layout < 0) layout = lay;
-        if (layout != lay) return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
+        // binary items with and without learned order share the ordered kernel
+        // (per-item order mask); the 8-wide layout cannot mix with them
+        if (layout < 0 || (layout == kBinary && lay == kOrdered)) layout = lay;
+        if (layout != lay && !(layout == kOrdered && lay == kBinary))
+            return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
         BatchItem& b = table.item[i];
         b.nodes = wide ? s->wide : s->nodes;
         b.tris = s->tris;
@@ -761,6 +769,7 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
         b.begin = (uint32_t)total;
         b.n = (uint32_t)it.n_rays;
         b.any = (it.flags & RTQ_TRACE_ANY) ? 1u : 0u;
+        b.learned = lay == kOrdered ? 1u : 0u;
         total += it.n_rays;
         if (total > 0x80000000ull) return fail(ctx, RTQ_ECAPACITY, "trace_batch: more than 2^31 rays in one launch");
         depth = std::max(depth, stack_bound(s, wide));

@@ -262,10 +262,26 @@ def test_learned_order_exact(rtq_ctx, occ, torch_mod):
     rtq_ctx.synchronize()
     assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d)
     assert np.array_equal(devmem.to_host(hs, HIT_DTYPE)["id"] >= 0, plain_s["id"] >= 0)
+    # learned and plain binary items share one launch (each keeps its own order)
     mixed = rtq_ctx.items([(sc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
                            (sc, occ["shad"].data_ptr(), 4 * n, hs.data_ptr(), RTQ_TRACE_ANY | RTQ_TRACE_BINARY)])
-    with pytest.raises(RtqError):
-        rtq_ctx.trace_batch(mixed)
+    rtq_ctx.trace_batch(mixed)
+    rtq_ctx.synchronize()
+    assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d)
+    assert np.array_equal(devmem.to_host(hs, HIT_DTYPE)["id"] >= 0, plain_s["id"] >= 0)
+    # ... but not with an 8-wide item
+    wide = rtq_ctx.upload(*scenes.soup_scene(2000, 3))
+    wide.build(16, 4, 0, node_width=8)
+    try:
+        bad = rtq_ctx.items([(sc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
+                             (wide, occ["diff"].data_ptr(), 1000, hs.data_ptr(), 0)])
+        with pytest.raises(RtqError):
+            rtq_ctx.trace_batch(bad)
+    finally:
+        wide.release()
+    # a mode without decisions traces like the plain kernel
+    sc.order_apply(model.params.data_ptr(), 0.5, 1.0)
+    assert hits_equal(trace(sc, occ["diff"], n, RTQ_TRACE_LEARNED), plain_d)
     # a rebuild clears the order bits
     sc.build(16, 4, 0)
     with pytest.raises(RtqError):

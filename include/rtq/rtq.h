@@ -288,7 +288,8 @@ rtq_status rtq_raygen_shadow(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* pri
                              uint32_t n_lights, float eps, rtq_ray* out_dev, void* stream);
 
 /* ---- learned child ordering (no counterpart in the reference: its
- * traversal always enters the left child first, raytracer.glsl:264-290) ----
+ * traversal pushes left then right, so it always enters the right child
+ * first, raytracer.glsl:283-289) ----
  *
  * A 32 -> 64 -> 64 -> 1 MLP (ReLU, ReLU, logistic) scores a binary node for a
  * ray octant (bit k set <=> direction component k < 0) and a mode (closest /

@@ -23,6 +23,12 @@ here too, `exact`):
               decisions only, closest-hit rays keep the nearer child)
 `value` is the fastest form; `detail.forms` has every form.
 
+Further workloads in `detail` (not the headline):
+`detail.compressed_8m`: four tiled terrains (8,372,232 triangles), 2048 x 2048
+primary + 4M diffuse rays, 8-wide compressed against binary nodes, hit
+records compared whole, t error against an FP64 ray/plane distance.
+`detail.uneven_batch`: 64 scenes of 1e3-4e6 triangles with 64k-2M rays each,
+one graph-replayed launch against 64 launches, results compared.
 `detail.occluded` measures the learned-ordering workload: the same terrain
 plus 2,097,152 random-soup triangles over the unit cube, 2048 x 2048 primary
 hits -> 4M diffuse + 16M shadow rays (nearly all occluded): plain, learned
@@ -65,6 +71,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-occluded", action="store_true", help="skip the terrain + soup learned-order workload")
     ap.add_argument("--occluded-res", type=int, default=2048, help="primary image side of the occluded workload")
+    ap.add_argument("--no-compressed-8m", action="store_true", help="skip the 4-terrain 8-wide workload")
+    ap.add_argument("--no-uneven-batch", action="store_true", help="skip the 64-scene uneven batch")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
@@ -166,6 +174,26 @@ def train_order(ctx, wl, stream):
             "loss_first": round(float(losses[0]), 4) if len(losses) else None,
             "loss_last": round(float(losses[-1]), 4) if len(losses) else None,
             "setup_s": round(time.perf_counter() - t0, 2)}
+
+
+def t_error_stats(tris, rays, hits, n_sample=20_000, seed=0):
+    """t of the hits against the FP64 plane distance of the hit triangle, in
+    ulps of the problem's scale max(|t|, |o|, |p0|): median, 99th percentile,
+    max, and max of the error times |cos(incidence)|."""
+    hit_idx = np.nonzero(hits["id"] >= 0)[0]
+    idx = np.random.default_rng(seed).choice(hit_idx, size=min(n_sample, len(hit_idx)), replace=False)
+    h, r = hits[idx], rays[idx]
+    tri = tris[h["id"]]
+    p0, p1, p2 = (tri[k][:, :3].astype(np.float64) for k in ("p0", "p1", "p2"))
+    o, d = r["o"].astype(np.float64), r["d"].astype(np.float64)
+    nrm = np.cross(p1 - p0, p2 - p0)
+    t64 = np.einsum("ij,ij->i", nrm, p0 - o) / np.einsum("ij,ij->i", nrm, d)
+    scale = np.maximum(np.abs(t64), np.maximum(np.abs(o).max(axis=1), np.abs(p0).max(axis=1)))
+    err = np.abs(h["t"].astype(np.float64) - t64) / np.spacing(scale.astype(np.float32)).astype(np.float64)
+    cos = np.abs(np.einsum("ij,ij->i", nrm, d)) / (np.linalg.norm(nrm, axis=1) * np.linalg.norm(d, axis=1))
+    return {"hits_sampled": int(len(err)), "median_ulp": round(float(np.median(err)), 3),
+            "p99_ulp": round(float(np.percentile(err, 99)), 3), "max_ulp": round(float(err.max()), 3),
+            "max_ulp_times_cos": round(float((err * cos).max()), 3)}
 
 
 def main() -> int:
@@ -373,6 +401,111 @@ def main() -> int:
             "setup_s": round(time.time() - t3, 1)}
         owl.release()
 
+    # config 3: four tiled terrains, compressed 8-wide nodes against binary
+    compressed_8m = None
+    if not args.no_compressed_8m:
+        from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE
+        t4 = time.time()
+        ttris = scenes.tiled_terrains(1024, 2, seed=0)
+        tsc = ctx.upload(ttris, scenes.identity_models())
+        tinfo = tsc.build(16, args.max_leaf, 0, node_width=8)
+        tcam = scenes.pinhole(**scenes.TILED_CAMERA)
+        n = tcam.width * tcam.height
+        prim, diff = devmem.empty(n, RAY_DTYPE), devmem.empty(n, RAY_DTYPE)
+        hit = {k: devmem.empty(n, HIT_DTYPE) for k in ("pb", "pw", "db", "dw")}
+        ctx.raygen_pinhole(tcam, prim.data_ptr(), sp)
+        tsc.trace(prim.data_ptr(), n, hit["pb"].data_ptr(), RTQ_TRACE_BINARY, sp)
+        tsc.raygen_diffuse(prim.data_ptr(), hit["pb"].data_ptr(), n, scenes.DIFFUSE_SEED, scenes.SECONDARY_EPS,
+                           diff.data_ptr(), sp)
+        torch.cuda.synchronize()
+        forms3 = {}
+        for lname, layout in (("binary", RTQ_TRACE_BINARY), ("wide", 0)):
+            hp, hd = hit["p" + lname[0]], hit["d" + lname[0]]
+
+            def step(ev=None, layout=layout, hp=hp, hd=hd):
+                if ev is not None:
+                    ev[0].record(stream)
+                tsc.trace(prim.data_ptr(), n, hp.data_ptr(), layout, sp)
+                if ev is not None:
+                    ev[1].record(stream)
+                tsc.trace(diff.data_ptr(), n, hd.data_ptr(), layout, sp)
+                if ev is not None:
+                    ev[2].record(stream)
+            ms_per_step, evs = time_steps(step)
+            forms3[lname] = {"Mrays_s": round(2 * n / (ms_per_step * 1e-3) / 1e6, 2),
+                             "ms_per_step": round(ms_per_step, 4),
+                             "primary_ms": round(float(np.mean([e[0].elapsed_time(e[1]) for e in evs])), 4),
+                             "diffuse_ms": round(float(np.mean([e[1].elapsed_time(e[2]) for e in evs])), 4)}
+        torch.cuda.synchronize()
+        exact = bool(torch.equal(hit["pb"], hit["pw"]) and torch.equal(hit["db"], hit["dw"]))
+        prim_h, diff_h = devmem.to_host(prim, RAY_DTYPE), devmem.to_host(diff, RAY_DTYPE)
+        compressed_8m = {
+            "workload": f"4 tiled terrains 1024x1024 ({tinfo.n_tris:,} tris), {tcam.width}x{tcam.height} primary "
+                        "rays + 1 diffuse ray per hit, closest hit",
+            "rays_per_step": 2 * n, "forms": forms3, "ids_exact_wide_vs_binary": exact,
+            "wide_vs_binary": round(forms3["wide"]["Mrays_s"] / forms3["binary"]["Mrays_s"], 3),
+            "t_error_vs_fp64": {"primary": t_error_stats(ttris, prim_h, devmem.to_host(hit["pw"], HIT_DTYPE)),
+                                "diffuse": t_error_stats(ttris, diff_h, devmem.to_host(hit["dw"], HIT_DTYPE))},
+            "build_ms": round(tinfo.build_ms, 2), "trav_nodes": tinfo.n_trav_nodes, "wide_nodes": tinfo.n_wide_nodes,
+            "setup_s": round(time.time() - t4, 1)}
+        del ttris, prim_h, diff_h
+        tsc.release()
+
+    # config 2: 64 uneven scenes (soups and spheres), one graph-replayed launch
+    # against one launch per scene
+    uneven = None
+    if not args.no_uneven_batch:
+        from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE
+        t5 = time.time()
+        ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide))
+        bsc, brays, bsep, bbat, bn = [], [], [], [], []
+        n_tris_total = 0
+        for s_ in range(64):
+            btris, rays = scenes.batched_scene(s_, seed=0)
+            sc_ = ctx.upload(btris, scenes.identity_models())
+            sc_.build(16, args.max_leaf, 0, node_width=8)
+            n_tris_total += len(btris)
+            bsc.append(sc_)
+            brays.append(devmem.to_device(rays))
+            bn.append(len(rays))
+            bsep.append(devmem.empty(len(rays), HIT_DTYPE))
+            bbat.append(devmem.empty(len(rays), HIT_DTYPE))
+            del btris, rays
+        torch.cuda.synchronize()
+        gen_s = time.time() - t5
+        total_rays = sum(bn)
+
+        def separate(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            for sc_, r_, h_, n_ in zip(bsc, brays, bsep, bn):
+                sc_.trace(r_.data_ptr(), n_, h_.data_ptr(), 0, sp)
+            if ev is not None:
+                ev[1].record(stream)
+                ev[2].record(stream)
+        sep_ms, _ = time_steps(separate)
+        items = ctx.items([(sc_, r_.data_ptr(), n_, h_.data_ptr(), 0) for sc_, r_, h_, n_ in zip(bsc, brays, bbat, bn)])
+        tr = ctx.autotune(items, reps=2)
+        graph = ctx.batch(items)
+        bat_ms, _ = time_steps(graph_step(graph))
+        graph.destroy()
+        torch.cuda.synchronize()
+        exact = all(torch.equal(a, b) for a, b in zip(bsep, bbat))
+        uneven = {
+            "workload": f"64 scenes, triangles log-uniform in [1e3, 4e6] ({n_tris_total:,} in all), soups and "
+                        f"jittered spheres, {total_rays:,} random-direction rays (log-uniform 64k-2M per scene), "
+                        "closest hit",
+            "separate_launches": {"Mrays_s": round(total_rays / (sep_ms * 1e-3) / 1e6, 2), "ms": round(sep_ms, 4)},
+            "batched_graph": {"Mrays_s": round(total_rays / (bat_ms * 1e-3) / 1e6, 2), "ms": round(bat_ms, 4),
+                              "tuned": {"refill": tr.best.refill, "chunk": tr.best.chunk,
+                                        "prefer_wide": tr.best.prefer_wide, "candidates": tr.n_tried}},
+            "batched_vs_separate": round(sep_ms / bat_ms, 3), "exact": exact,
+            "generation_s": round(gen_s, 1), "setup_s": round(time.time() - t5, 1)}
+        for sc_ in bsc:
+            sc_.release()
+        del brays, bsep, bbat
+        ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide))
+
     def rounded(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
 
@@ -403,6 +536,8 @@ def main() -> int:
                    if "learned" in results and "plain" in results else None,
                    "order_model": learned_setup,
                    "occluded": occluded,
+                   "compressed_8m": compressed_8m,
+                   "uneven_batch": uneven,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
                    "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
                    "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},

@@ -221,24 +221,46 @@ def random_rays(n: int, seed: int, lo, hi) -> np.ndarray:
     return rays
 
 
+def batched_scene(s: int, seed: int = 0, tri_range=(1_000, 4_000_000), ray_range=(65_536, 2_097_152)):
+    """Scene s of the uneven batch: a triangle count log-uniform in tri_range,
+    random soup for even s and a jittered sphere for odd s, with a log-uniform
+    number of random-direction rays from inside its bounds.  Returns
+    (triangles, rays)."""
+    lt = math.log(tri_range[0]) + float(u01(seed, 20, s)) * (math.log(tri_range[1]) - math.log(tri_range[0]))
+    lr = math.log(ray_range[0]) + float(u01(seed, 21, s)) * (math.log(ray_range[1]) - math.log(ray_range[0]))
+    n_tris, n_rays = int(math.exp(lt)), int(math.exp(lr))
+    if s % 2 == 0:
+        tris = soup(n_tris, seed=seed * 1000 + s)
+        lo, hi = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    else:
+        tris = sphere_with(n_tris, seed=seed * 1000 + s)
+        lo, hi = (-0.9, -0.9, -0.9), (0.9, 0.9, 0.9)
+    return tris, random_rays(n_rays, seed * 1000 + s, lo, hi)
+
+
 def batched_workload(n_scenes: int = 64, seed: int = 0, tri_range=(1_000, 4_000_000), ray_range=(65_536, 2_097_152)):
-    """Uneven scenes for one batched launch: triangle counts log-uniform in
-    tri_range, alternating random soup and jittered spheres, each with a
-    log-uniform number of random-direction rays from inside its bounds.
-    Returns a list of (triangles, rays)."""
+    """The uneven batch (config 2 at the default ranges) as a list of (triangles, rays)."""
+    return [batched_scene(s, seed, tri_range, ray_range) for s in range(n_scenes)]
+
+
+def tiled_terrains(res: int = 1024, tiles: int = 2, seed: int = 0) -> np.ndarray:
+    """tiles x tiles terrains of res x res vertices side by side over
+    [0, tiles]^2 in x and z, tile (a, b) with noise seed seed + a * tiles + b,
+    so neighbouring tiles differ (jittered heights) and share no vertices."""
     out = []
-    for s in range(n_scenes):
-        lt = math.log(tri_range[0]) + u01(seed, 20, s) * (math.log(tri_range[1]) - math.log(tri_range[0]))
-        lr = math.log(ray_range[0]) + u01(seed, 21, s) * (math.log(ray_range[1]) - math.log(ray_range[0]))
-        n_tris, n_rays = int(math.exp(lt)), int(math.exp(lr))
-        if s % 2 == 0:
-            tris = soup(n_tris, seed=seed * 1000 + s)
-            lo, hi = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
-        else:
-            tris = sphere_with(n_tris, seed=seed * 1000 + s)
-            lo, hi = (-0.9, -0.9, -0.9), (0.9, 0.9, 0.9)
-        out.append((tris, random_rays(n_rays, seed * 1000 + s, lo, hi)))
-    return out
+    for a in range(tiles):
+        for b in range(tiles):
+            t = terrain(res, seed + a * tiles + b)
+            for v in ("p0", "p1", "p2"):
+                t[v][:, 0] += a
+                t[v][:, 2] += b
+            out.append(t)
+    return np.concatenate(out)
+
+
+# Camera over the 2 x 2 tiled terrains: every primary ray hits
+TILED_CAMERA = dict(eye=(1.0, 1.4, 0.2), target=(1.0, 0.0, 1.0), up=(0.0, 1.0, 0.0), vfov_deg=40.0,
+                    width=2048, height=2048)
 
 
 def soup_scene(n: int = 65536, seed: int = 0):

@@ -230,7 +230,9 @@ typedef struct rtq_launch_params {
     uint32_t refill;       /* refill a wave's idle lanes once this many (1..64) are idle */
     uint32_t chunk;        /* consecutive rays a wave takes per atomic (1..4096) */
     uint32_t prefer_wide;  /* trace 8-wide nodes when a scene has them */
-    uint32_t reserved;
+    uint32_t spec_slack;   /* speculative traversal: a wave switches from node visits to
+                              leaf tests once at most this many (0..63) of its lanes
+                              are still looking for their first leaf (0: none) */
 } rtq_launch_params;
 
 rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out);
@@ -244,8 +246,8 @@ typedef struct rtq_tune_result {
     uint32_t from_cache;
 } rtq_tune_result;
 
-/* Times candidate launch settings (refill, chunk, node layout) on the given
- * batch, `reps` launches each, applies the fastest to the context and records
+/* Times candidate launch settings (refill, chunk, speculation slack, node
+ * layout) on the given batch, `reps` launches each, applies the fastest to the context and records
  * it under a key of (device, scene sizes, ray counts, modes); a later call
  * with the same key reuses it without timing.  Writes the items' hit buffers. */
 rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, uint32_t reps,

@@ -279,6 +279,10 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
         const int v = atoi(e);
         if (v >= 1 && v <= 4096) ctx->chunk = (uint32_t)v;
     }
+    if (const char* e = getenv("RTQ_SPEC_SLACK")) {
+        const int v = atoi(e);
+        if (v >= 0 && v <= 63) ctx->spec_slack = (uint32_t)v;
+    }
     *out = ctx;
     return RTQ_OK;
 }

@@ -38,13 +38,15 @@ uint64_t tune_key(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n) {
 }
 
 bool valid_params(const rtq_launch_params& p) {
-    return p.refill >= 1 && p.refill <= 64 && p.chunk >= 1 && p.chunk <= 4096 && p.prefer_wide <= 1;
+    return p.refill >= 1 && p.refill <= 64 && p.chunk >= 1 && p.chunk <= 4096 && p.prefer_wide <= 1 &&
+           p.spec_slack <= 63;
 }
 
 void apply(rtq_ctx* ctx, const rtq_launch_params& p) {
     ctx->refill = p.refill;
     ctx->chunk = p.chunk;
     ctx->prefer_wide = p.prefer_wide != 0;
+    ctx->spec_slack = p.spec_slack;
 }
 
 rtq_launch_params current(const rtq_ctx* ctx) {
@@ -52,11 +54,12 @@ rtq_launch_params current(const rtq_ctx* ctx) {
     p.refill = ctx->refill;
     p.chunk = ctx->chunk;
     p.prefer_wide = ctx->prefer_wide ? 1u : 0u;
+    p.spec_slack = ctx->spec_slack;
     return p;
 }
 
 constexpr uint32_t kTuneMagic = 0x4e555452u;  // "RTUN"
-constexpr uint32_t kTuneVersion = 1;
+constexpr uint32_t kTuneVersion = 2;
 
 struct TuneRecord {
     uint64_t key;
@@ -133,7 +136,7 @@ rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out) {
 
 rtq_status rtq_launch_params_set(rtq_ctx* ctx, const rtq_launch_params* p) {
     if (!ctx || !p) return RTQ_EINVAL;
-    if (!valid_params(*p)) return fail(ctx, RTQ_EINVAL, "launch_params: refill 1..64, chunk 1..4096, prefer_wide 0|1");
+    if (!valid_params(*p)) return fail(ctx, RTQ_EINVAL, "launch_params: refill 1..64, chunk 1..4096, prefer_wide 0|1, spec_slack 0..63");
     apply(ctx, *p);
     return RTQ_OK;
 }
@@ -182,22 +185,27 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
     rtq_launch_params best = before;
     float best_ms = base_ms;
     uint32_t tried = 1;
-    const uint32_t refills[] = {4, 8, 16, 32, 48};
-    const uint32_t chunks[] = {64, 128, 256, 512};
+    const uint32_t refills[] = {4, 8, 16, 32};
+    const uint32_t chunks[] = {128, 256, 512};
+    const uint32_t slacks[] = {0, 4, 8, 16};
     for (uint32_t wide = 0; wide <= (any_wide ? 1u : 0u) && status == RTQ_OK; ++wide) {
         for (uint32_t rf : refills) {
             for (uint32_t ch : chunks) {
-                rtq_launch_params p{rf, ch, wide, 0};
-                if (p.refill == before.refill && p.chunk == before.chunk && p.prefer_wide == before.prefer_wide)
-                    continue;
-                float ms = 0.0f;
-                status = time_params(p, &ms);
-                if (status != RTQ_OK) break;
-                ++tried;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = p;
+                for (uint32_t sl : slacks) {
+                    rtq_launch_params p{rf, ch, wide, sl};
+                    if (p.refill == before.refill && p.chunk == before.chunk && p.prefer_wide == before.prefer_wide &&
+                        p.spec_slack == before.spec_slack)
+                        continue;
+                    float ms = 0.0f;
+                    status = time_params(p, &ms);
+                    if (status != RTQ_OK) break;
+                    ++tried;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = p;
+                    }
                 }
+                if (status != RTQ_OK) break;
             }
             if (status != RTQ_OK) break;
         }

@@ -96,6 +96,8 @@ struct rtq_ctx {
     // 8-wide nodes when a scene has them and prefer_wide is set
     uint32_t refill = 16;
     uint32_t chunk = 128;
+    // leaf tests start once at most spec_slack lanes are still descending
+    uint32_t spec_slack = 4;
     bool prefer_wide = true;
     // autotuner results: key -> launch settings
     std::map<uint64_t, rtq_launch_params> tuned;

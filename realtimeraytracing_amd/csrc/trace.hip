@@ -91,7 +91,8 @@ struct TraceArgs {
     uint32_t* spill;
     uint32_t spill_stride;
     uint32_t refill;     // refill once this many lanes of a wave are idle
-    uint32_t chunk;      // rays taken from the global counter per atomic
+    uint32_t chunk;      // rays taken from a work counter per atomic
+    uint32_t slack;      // leaf tests start once at most this many lanes still descend
     uint32_t* stat_nodes;
     uint32_t* stat_tris;
 };
@@ -533,9 +534,11 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
         // ---- traversal steps until enough lanes are idle
         for (;;) {
             if (rid >= 0) {
-                // Internal nodes until every lane of the wave holds a leaf: a lane
-                // that finds one postpones it and keeps descending (speculative
-                // traversal, Aila & Laine, HPG 2009).
+                // Internal nodes until (nearly) every lane of the wave holds a
+                // leaf: a lane that finds one postpones it and keeps descending
+                // (speculative traversal, Aila & Laine, HPG 2009); the wave
+                // moves on to leaf tests once at most `slack` lanes still look
+                // for their first leaf.
                 bool searching = leaf == kEmpty;
                 while (is_node_ref(cur)) {
                     visit();
@@ -544,44 +547,47 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                         cur = pop();
                         searching = false;
                     }
-                    if (!__ballot(searching)) break;
+                    if ((uint32_t)__popcll(__ballot(searching)) <= a.slack) break;
                 }
-                // postponed leaf, then any leaf that is waiting in cur
+                // postponed leaf, then any leaf that is waiting in cur: one
+                // triangle per lane per iteration, so a lane whose leaf ends
+                // early goes on with its next leaf instead of waiting for the
+                // longest leaf of the wave
+                uint32_t k = 0;
                 while (is_leaf_ref(leaf)) {
                     const uint32_t first = (leaf >> 3) & 0x0FFFFFFFu;
                     const uint32_t count = (leaf & 7u) + 1u;
-                    bool done = false;
-                    for (uint32_t k = 0; k < count; ++k) {
-                        const float4* tp = tris_p + 3ull * (first + k);
-                        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-                        if (STATS) ++n_tests;
-                        const int id = (int)__float_as_uint(p0.w);
-                        float t, ua, ub, uc = 0.0f;
-                        bool h;
-                        if (LEAF == 0)
-                            h = wt_test(p0, p1, p2, kx, ky, kz, opx, opy, opz, sx, sy, sz, tmin, tbest, t, ua, ub, uc);
-                        else
-                            h = refmt_test(p0, p1, p2, ox, oy, oz, dx, dy, dz, tmin, tbest, t, ua, ub);
-                        if (h && (best_id < 0 || t < best_t || (t == best_t && id < best_id))) {
-                            best_id = id;
-                            best_t = t;
-                            best_a = ua;
-                            best_b = ub;
-                            best_c = uc;
-                            tbest = t;
-                            if (MODE == 1 || (MODE == 2 && lane_any)) { done = true; break; }
+                    const float4* tp = tris_p + 3ull * (first + k);
+                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                    if (STATS) ++n_tests;
+                    const int id = (int)__float_as_uint(p0.w);
+                    float t, ua, ub, uc = 0.0f;
+                    bool h;
+                    if (LEAF == 0)
+                        h = wt_test(p0, p1, p2, kx, ky, kz, opx, opy, opz, sx, sy, sz, tmin, tbest, t, ua, ub, uc);
+                    else
+                        h = refmt_test(p0, p1, p2, ox, oy, oz, dx, dy, dz, tmin, tbest, t, ua, ub);
+                    if (h && (best_id < 0 || t < best_t || (t == best_t && id < best_id))) {
+                        best_id = id;
+                        best_t = t;
+                        best_a = ua;
+                        best_b = ub;
+                        best_c = uc;
+                        tbest = t;
+                        if (MODE == 1 || (MODE == 2 && lane_any)) {
+                            cur = kEmpty;
+                            leaf = kEmpty;
+                            break;
                         }
                     }
-                    if (done) {
-                        cur = kEmpty;
-                        leaf = kEmpty;
-                        break;
-                    }
-                    if (is_leaf_ref(cur)) {
-                        leaf = cur;
-                        cur = pop();
-                    } else {
-                        leaf = kEmpty;
+                    if (++k == count) {
+                        k = 0;
+                        if (is_leaf_ref(cur)) {
+                            leaf = cur;
+                            cur = pop();
+                        } else {
+                            leaf = kEmpty;
+                        }
                     }
                 }
                 if (cur == kEmpty && leaf == kEmpty) finish();
@@ -645,6 +651,7 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
     args.spill_stride = (uint32_t)threads;
     args.refill = ctx->refill;
     args.chunk = ctx->chunk;
+    args.slack = ctx->spec_slack;
     hipLaunchKernelGGL((k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st,
                        args, table);
     RTQ_HIP(ctx, hipGetLastError());

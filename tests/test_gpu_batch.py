@@ -126,7 +126,8 @@ def test_autotune_cache_roundtrip(rtq_ctx, tmp_path):
         assert res.n_tried > 10 and res.from_cache == 0
         assert res.best_ms <= res.default_ms
         got = rtq_ctx.launch_params()
-        assert (got.refill, got.chunk, got.prefer_wide) == (res.best.refill, res.best.chunk, res.best.prefer_wide)
+        assert (got.refill, got.chunk, got.prefer_wide, got.spec_slack) == (
+            res.best.refill, res.best.chunk, res.best.prefer_wide, res.best.spec_slack)
         check_same(wl.hits(), wl.solo(), wl.flags)         # tuned settings give the same hits
         again = rtq_ctx.autotune(wl.items(), reps=1)
         assert again.from_cache == 1 and again.n_tried == 0
@@ -144,7 +145,7 @@ def test_autotune_cache_roundtrip(rtq_ctx, tmp_path):
         bad.write_bytes(bytes(data))
         with pytest.raises(RtqError):
             rtq_ctx.tune_load(bad)
-        rtq_ctx.set_launch_params(before.refill, before.chunk, bool(before.prefer_wide))
+        rtq_ctx.set_launch_params(before.refill, before.chunk, bool(before.prefer_wide), before.spec_slack)
     finally:
         wl.release()
 
@@ -155,4 +156,22 @@ def test_launch_params_validation(rtq_ctx):
         rtq_ctx.set_launch_params(0, 128)
     with pytest.raises(RtqError):
         rtq_ctx.set_launch_params(8, 5000)
-    rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide))
+    with pytest.raises(RtqError):
+        rtq_ctx.set_launch_params(8, 128, True, 64)
+    rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack)
+
+
+@pytest.mark.parametrize("slack", [0, 1, 16, 63])
+def test_spec_slack_same_hits(rtq_ctx, slack):
+    """The speculation slack only changes when leaves are tested."""
+    wl = Workload(rtq_ctx, 4, seed=11)
+    p = rtq_ctx.launch_params()
+    try:
+        want = wl.solo()
+        rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), slack)
+        check_same(wl.solo(), want, wl.flags)
+        rtq_ctx.trace_batch(wl.items())
+        check_same(wl.hits(), want, wl.flags)
+    finally:
+        rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack)
+        wl.release()

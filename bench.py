@@ -265,7 +265,7 @@ def main() -> int:
     learned_setup = None
     for form in args.forms.split(","):
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack)
+                              default_params.spec_slack, default_params.leaf_slack)
         tuned = None
         if form in ("plain", "compressed"):
             layout = RTQ_TRACE_BINARY if form == "plain" else 0
@@ -290,7 +290,7 @@ def main() -> int:
                                (wl.sc, wl.shad.data_ptr(), wl.n_shad, wl.hs.data_ptr(), RTQ_TRACE_ANY | extra)])
             tr = ctx.autotune(items, reps=2)
             tuned = {"refill": tr.best.refill, "chunk": tr.best.chunk, "prefer_wide": tr.best.prefer_wide,
-                     "spec_slack": tr.best.spec_slack, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
+                     "spec_slack": tr.best.spec_slack, "leaf_slack": tr.best.leaf_slack, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
                      "default_ms": round(tr.default_ms, 4)}
             lname = "learned" if form == "learned" else ("wide" if tr.best.prefer_wide else "binary")
             graph = ctx.batch(items)
@@ -320,7 +320,7 @@ def main() -> int:
                          "bytes_per_ray": "32 ray + 16 hit + %d x node visits + 48 x triangle tests" % node_bytes},
         }
     ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack)
+                              default_params.spec_slack, default_params.leaf_slack)
 
     best = max(results, key=lambda f: results[f]["Mrays_s"])
     res = results[best]
@@ -372,7 +372,7 @@ def main() -> int:
                 flags = RTQ_TRACE_LEARNED
             if form == "learned_batched":
                 ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack)
+                              default_params.spec_slack, default_params.leaf_slack)
                 items = ctx.items([(owl.sc, owl.diff.data_ptr(), owl.n_diff, owl.hd.data_ptr(), flags),
                                    (owl.sc, owl.shad.data_ptr(), owl.n_shad, owl.hs.data_ptr(),
                                     RTQ_TRACE_ANY | flags)])
@@ -462,7 +462,7 @@ def main() -> int:
         from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE
         t5 = time.time()
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack)
+                              default_params.spec_slack, default_params.leaf_slack)
         bsc, brays, bsep, bbat, bn = [], [], [], [], []
         n_tris_total = 0
         for s_ in range(64):
@@ -504,6 +504,7 @@ def main() -> int:
             "batched_graph": {"Mrays_s": round(total_rays / (bat_ms * 1e-3) / 1e6, 2), "ms": round(bat_ms, 4),
                               "tuned": {"refill": tr.best.refill, "chunk": tr.best.chunk,
                                         "prefer_wide": tr.best.prefer_wide, "spec_slack": tr.best.spec_slack,
+                                        "leaf_slack": tr.best.leaf_slack,
                                         "candidates": tr.n_tried}},
             "batched_vs_separate": round(sep_ms / bat_ms, 3), "exact": exact,
             "generation_s": round(gen_s, 1), "setup_s": round(time.time() - t5, 1)}
@@ -511,7 +512,7 @@ def main() -> int:
             sc_.release()
         del brays, bsep, bbat
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack)
+                              default_params.spec_slack, default_params.leaf_slack)
 
     def rounded(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}

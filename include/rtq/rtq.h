@@ -233,6 +233,10 @@ typedef struct rtq_launch_params {
     uint32_t spec_slack;   /* speculative traversal: a wave switches from node visits to
                               leaf tests once at most this many (0..63) of its lanes
                               are still looking for their first leaf (0: none) */
+    uint32_t leaf_slack;   /* ... and back to node visits once at most this many (0..63)
+                              lanes still have triangles to test; those lanes resume
+                              their leaf in the next leaf phase (0: none) */
+    uint32_t reserved;     /* 0 */
 } rtq_launch_params;
 
 rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out);

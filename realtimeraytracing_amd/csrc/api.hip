@@ -283,6 +283,10 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
         const int v = atoi(e);
         if (v >= 0 && v <= 63) ctx->spec_slack = (uint32_t)v;
     }
+    if (const char* e = getenv("RTQ_LEAF_SLACK")) {
+        const int v = atoi(e);
+        if (v >= 0 && v <= 63) ctx->leaf_slack = (uint32_t)v;
+    }
     *out = ctx;
     return RTQ_OK;
 }

@@ -39,7 +39,7 @@ uint64_t tune_key(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n) {
 
 bool valid_params(const rtq_launch_params& p) {
     return p.refill >= 1 && p.refill <= 64 && p.chunk >= 1 && p.chunk <= 4096 && p.prefer_wide <= 1 &&
-           p.spec_slack <= 63;
+           p.spec_slack <= 63 && p.leaf_slack <= 63 && p.reserved == 0;
 }
 
 void apply(rtq_ctx* ctx, const rtq_launch_params& p) {
@@ -47,6 +47,7 @@ void apply(rtq_ctx* ctx, const rtq_launch_params& p) {
     ctx->chunk = p.chunk;
     ctx->prefer_wide = p.prefer_wide != 0;
     ctx->spec_slack = p.spec_slack;
+    ctx->leaf_slack = p.leaf_slack;
 }
 
 rtq_launch_params current(const rtq_ctx* ctx) {
@@ -55,11 +56,12 @@ rtq_launch_params current(const rtq_ctx* ctx) {
     p.chunk = ctx->chunk;
     p.prefer_wide = ctx->prefer_wide ? 1u : 0u;
     p.spec_slack = ctx->spec_slack;
+    p.leaf_slack = ctx->leaf_slack;
     return p;
 }
 
 constexpr uint32_t kTuneMagic = 0x4e555452u;  // "RTUN"
-constexpr uint32_t kTuneVersion = 2;
+constexpr uint32_t kTuneVersion = 3;
 
 struct TuneRecord {
     uint64_t key;
@@ -136,7 +138,7 @@ rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out) {
 
 rtq_status rtq_launch_params_set(rtq_ctx* ctx, const rtq_launch_params* p) {
     if (!ctx || !p) return RTQ_EINVAL;
-    if (!valid_params(*p)) return fail(ctx, RTQ_EINVAL, "launch_params: refill 1..64, chunk 1..4096, prefer_wide 0|1, spec_slack 0..63");
+    if (!valid_params(*p)) return fail(ctx, RTQ_EINVAL, "launch_params: refill 1..64, chunk 1..4096, prefer_wide 0|1, slacks 0..63, reserved 0");
     apply(ctx, *p);
     return RTQ_OK;
 }
@@ -185,31 +187,34 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
     rtq_launch_params best = before;
     float best_ms = base_ms;
     uint32_t tried = 1;
+    // stage 1: refill x chunk x node layout with the slacks in force; stage 2:
+    // both slacks with the best stage-1 settings
+    auto consider = [&](const rtq_launch_params& p) -> rtq_status {
+        if (memcmp(&p, &before, sizeof p) == 0) return RTQ_OK;
+        float ms = 0.0f;
+        const rtq_status s = time_params(p, &ms);
+        if (s != RTQ_OK) return s;
+        ++tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = p;
+        }
+        return RTQ_OK;
+    };
     const uint32_t refills[] = {4, 8, 16, 32};
     const uint32_t chunks[] = {128, 256, 512};
-    const uint32_t slacks[] = {0, 4, 8, 16};
-    for (uint32_t wide = 0; wide <= (any_wide ? 1u : 0u) && status == RTQ_OK; ++wide) {
-        for (uint32_t rf : refills) {
-            for (uint32_t ch : chunks) {
-                for (uint32_t sl : slacks) {
-                    rtq_launch_params p{rf, ch, wide, sl};
-                    if (p.refill == before.refill && p.chunk == before.chunk && p.prefer_wide == before.prefer_wide &&
-                        p.spec_slack == before.spec_slack)
-                        continue;
-                    float ms = 0.0f;
-                    status = time_params(p, &ms);
-                    if (status != RTQ_OK) break;
-                    ++tried;
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = p;
-                    }
-                }
-                if (status != RTQ_OK) break;
-            }
-            if (status != RTQ_OK) break;
-        }
-    }
+    for (uint32_t wide = 0; wide <= (any_wide ? 1u : 0u) && status == RTQ_OK; ++wide)
+        for (uint32_t rf : refills)
+            for (uint32_t ch : chunks)
+                if (status == RTQ_OK)
+                    status = consider(rtq_launch_params{rf, ch, wide, before.spec_slack, before.leaf_slack, 0});
+    const rtq_launch_params stage1 = best;
+    const uint32_t spec[] = {0, 4, 8, 16};
+    const uint32_t leaf[] = {0, 8, 16};
+    for (uint32_t sl : spec)
+        for (uint32_t ll : leaf)
+            if (status == RTQ_OK && !(sl == stage1.spec_slack && ll == stage1.leaf_slack))
+                status = consider(rtq_launch_params{stage1.refill, stage1.chunk, stage1.prefer_wide, sl, ll, 0});
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (status != RTQ_OK) {

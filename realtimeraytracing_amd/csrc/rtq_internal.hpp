@@ -98,6 +98,8 @@ struct rtq_ctx {
     uint32_t chunk = 128;
     // leaf tests start once at most spec_slack lanes are still descending
     uint32_t spec_slack = 4;
+    // node visits resume once at most leaf_slack lanes still test triangles
+    uint32_t leaf_slack = 8;
     bool prefer_wide = true;
     // autotuner results: key -> launch settings
     std::map<uint64_t, rtq_launch_params> tuned;

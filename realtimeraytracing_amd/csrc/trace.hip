@@ -93,6 +93,7 @@ struct TraceArgs {
     uint32_t refill;     // refill once this many lanes of a wave are idle
     uint32_t chunk;      // rays taken from a work counter per atomic
     uint32_t slack;      // leaf tests start once at most this many lanes still descend
+    uint32_t leaf_slack; // node visits resume once at most this many lanes test triangles
     uint32_t* stat_nodes;
     uint32_t* stat_tris;
 };
@@ -261,6 +262,7 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
     // cur: next node to visit (or a leaf waiting behind the postponed one);
     // leaf: postponed leaf; kEmpty in both once the stack ran out
     uint32_t cur = kEmpty, leaf = kEmpty, sp = 0;
+    uint32_t leaf_k = 0;  // next triangle of `leaf`
     uint32_t n_nodes = 0, n_tests = 0;
 
     auto push = [&](uint32_t v) {
@@ -496,6 +498,7 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                     sp = 0;
                     cur = 0;
                     leaf = kEmpty;
+                    leaf_k = 0;
                     // box-test reciprocals (zero components replaced by +-1e-30)
                     const float ex = fabsf(dx) < 1e-30f ? copysignf(1e-30f, dx) : dx;
                     const float ey = fabsf(dy) < 1e-30f ? copysignf(1e-30f, dy) : dy;
@@ -552,12 +555,13 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                 // postponed leaf, then any leaf that is waiting in cur: one
                 // triangle per lane per iteration, so a lane whose leaf ends
                 // early goes on with its next leaf instead of waiting for the
-                // longest leaf of the wave
-                uint32_t k = 0;
+                // longest leaf of the wave; once at most `leaf_slack` lanes
+                // still test triangles the wave returns to node visits and
+                // those lanes resume their leaf (leaf, leaf_k) next time
                 while (is_leaf_ref(leaf)) {
                     const uint32_t first = (leaf >> 3) & 0x0FFFFFFFu;
                     const uint32_t count = (leaf & 7u) + 1u;
-                    const float4* tp = tris_p + 3ull * (first + k);
+                    const float4* tp = tris_p + 3ull * (first + leaf_k);
                     const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
                     if (STATS) ++n_tests;
                     const int id = (int)__float_as_uint(p0.w);
@@ -580,8 +584,8 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                             break;
                         }
                     }
-                    if (++k == count) {
-                        k = 0;
+                    if (++leaf_k == count) {
+                        leaf_k = 0;
                         if (is_leaf_ref(cur)) {
                             leaf = cur;
                             cur = pop();
@@ -589,6 +593,7 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
                             leaf = kEmpty;
                         }
                     }
+                    if ((uint32_t)__popcll(__ballot(is_leaf_ref(leaf))) <= a.leaf_slack) break;
                 }
                 if (cur == kEmpty && leaf == kEmpty) finish();
             }
@@ -652,6 +657,7 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
     args.refill = ctx->refill;
     args.chunk = ctx->chunk;
     args.slack = ctx->spec_slack;
+    args.leaf_slack = ctx->leaf_slack;
     hipLaunchKernelGGL((k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st,
                        args, table);
     RTQ_HIP(ctx, hipGetLastError());

@@ -62,7 +62,7 @@ class BatchItem(ctypes.Structure):
 
 class LaunchParams(ctypes.Structure):
     _fields_ = [("refill", ctypes.c_uint32), ("chunk", ctypes.c_uint32), ("prefer_wide", ctypes.c_uint32),
-                ("spec_slack", ctypes.c_uint32)]
+                ("spec_slack", ctypes.c_uint32), ("leaf_slack", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class TuneResult(ctypes.Structure):
@@ -241,8 +241,9 @@ class Context:
         self.check(self.lib.rtq_launch_params_get(self.h, ctypes.byref(p)))
         return p
 
-    def set_launch_params(self, refill: int, chunk: int, prefer_wide: bool = True, spec_slack: int = 4) -> None:
-        p = LaunchParams(refill, chunk, int(prefer_wide), spec_slack)
+    def set_launch_params(self, refill: int, chunk: int, prefer_wide: bool = True, spec_slack: int = 4,
+                          leaf_slack: int = 8) -> None:
+        p = LaunchParams(refill, chunk, int(prefer_wide), spec_slack, leaf_slack, 0)
         self.check(self.lib.rtq_launch_params_set(self.h, ctypes.byref(p)))
 
     def autotune(self, items, reps: int = 3) -> TuneResult:

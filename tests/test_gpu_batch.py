@@ -126,8 +126,8 @@ def test_autotune_cache_roundtrip(rtq_ctx, tmp_path):
         assert res.n_tried > 10 and res.from_cache == 0
         assert res.best_ms <= res.default_ms
         got = rtq_ctx.launch_params()
-        assert (got.refill, got.chunk, got.prefer_wide, got.spec_slack) == (
-            res.best.refill, res.best.chunk, res.best.prefer_wide, res.best.spec_slack)
+        assert (got.refill, got.chunk, got.prefer_wide, got.spec_slack, got.leaf_slack) == (
+            res.best.refill, res.best.chunk, res.best.prefer_wide, res.best.spec_slack, res.best.leaf_slack)
         check_same(wl.hits(), wl.solo(), wl.flags)         # tuned settings give the same hits
         again = rtq_ctx.autotune(wl.items(), reps=1)
         assert again.from_cache == 1 and again.n_tried == 0
@@ -145,7 +145,8 @@ def test_autotune_cache_roundtrip(rtq_ctx, tmp_path):
         bad.write_bytes(bytes(data))
         with pytest.raises(RtqError):
             rtq_ctx.tune_load(bad)
-        rtq_ctx.set_launch_params(before.refill, before.chunk, bool(before.prefer_wide), before.spec_slack)
+        rtq_ctx.set_launch_params(before.refill, before.chunk, bool(before.prefer_wide), before.spec_slack,
+                                  before.leaf_slack)
     finally:
         wl.release()
 
@@ -158,20 +159,24 @@ def test_launch_params_validation(rtq_ctx):
         rtq_ctx.set_launch_params(8, 5000)
     with pytest.raises(RtqError):
         rtq_ctx.set_launch_params(8, 128, True, 64)
-    rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack)
+    with pytest.raises(RtqError):
+        rtq_ctx.set_launch_params(8, 128, True, 4, 64)
+    rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack, p.leaf_slack)
 
 
-@pytest.mark.parametrize("slack", [0, 1, 16, 63])
-def test_spec_slack_same_hits(rtq_ctx, slack):
-    """The speculation slack only changes when leaves are tested."""
-    wl = Workload(rtq_ctx, 4, seed=11)
+@pytest.mark.parametrize("spec,leaf", [(0, 0), (1, 63), (16, 1), (63, 8), (4, 8)])
+def test_slacks_same_hits(rtq_ctx, spec, leaf):
+    """The two slacks only change when nodes and leaves are visited."""
+    wl = Workload(rtq_ctx, 4, seed=11, node_width=8)
     p = rtq_ctx.launch_params()
     try:
+        rtq_ctx.set_launch_params(p.refill, p.chunk, False, 0, 0)
         want = wl.solo()
-        rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), slack)
-        check_same(wl.solo(), want, wl.flags)
-        rtq_ctx.trace_batch(wl.items())
-        check_same(wl.hits(), want, wl.flags)
+        for wide in (False, True):
+            rtq_ctx.set_launch_params(p.refill, p.chunk, wide, spec, leaf)
+            check_same(wl.solo(), want, wl.flags)
+            rtq_ctx.trace_batch(wl.items())
+            check_same(wl.hits(), want, wl.flags)
     finally:
-        rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack)
+        rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack, p.leaf_slack)
         wl.release()

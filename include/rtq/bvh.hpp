@@ -6,6 +6,9 @@
 // leaf, parent and child arrays plus the sorted triangle indices.  preorder()
 // returns what Scene::getBVH_NodesToGPUData builds (scene.cpp:189-208).
 //
+// FrameBatch, autotune and OrderModel wrap the batched-launch, autotuner and
+// learned-ordering entry points the same way.
+//
 // Triangle and model element types only need the reference byte layouts
 // (64-B TriangleGPU, 68-B MeshModelGPU), so vectors of the reference's own glm
 // structs can be passed unchanged.  Failures throw rtq::Error (the reference
@@ -142,6 +145,80 @@ class BVH {
 
     std::shared_ptr<Context> ctx_;
     uint32_t scene_ = 0;
+};
+
+// A frame's ray sets (any scenes of one context) as one persistent launch
+// captured in a hipGraph; launch() replays it.  Item buffers must outlive it.
+class FrameBatch {
+  public:
+    FrameBatch(std::shared_ptr<Context> ctx, const std::vector<rtq_batch_item>& items) : ctx_(std::move(ctx)) {
+        rtq_batch* b = nullptr;
+        ctx_->check(rtq_batch_create(ctx_->get(), items.data(), static_cast<uint32_t>(items.size()), &b));
+        batch_.reset(b);
+    }
+    void launch(void* stream = nullptr) const { ctx_->check(rtq_batch_launch(batch_.get(), stream)); }
+
+  private:
+    struct Del {
+        void operator()(rtq_batch* b) const { rtq_batch_destroy(b); }
+    };
+    std::shared_ptr<Context> ctx_;
+    std::unique_ptr<rtq_batch, Del> batch_;
+};
+
+// Launch settings for these items from the autotuner (cached per workload).
+inline rtq_tune_result autotune(const Context& ctx, const std::vector<rtq_batch_item>& items, uint32_t reps = 2) {
+    rtq_tune_result r{};
+    ctx.check(rtq_autotune(ctx.get(), items.data(), static_cast<uint32_t>(items.size()), reps, &r));
+    return r;
+}
+
+// Learned child ordering: logs from a scene's rays, GPU training, order
+// words applied to a scene (rtq_order_model_*).  Traces then opt in with
+// RTQ_TRACE_LEARNED; results are those of the plain traversal.
+class OrderModel {
+  public:
+    explicit OrderModel(std::shared_ptr<Context> ctx, uint64_t seed = 1, uint64_t sample_capacity = 1u << 22)
+        : ctx_(std::move(ctx)) {
+        rtq_order_model* m = nullptr;
+        ctx_->check(rtq_order_model_create(ctx_->get(), seed, sample_capacity, &m));
+        model_.reset(m);
+    }
+    // samples from every stride-th device ray (flags: RTQ_TRACE_ANY or closest hit)
+    uint64_t log(const BVH& bvh, const rtq_ray* rays_dev, uint64_t n_rays, uint32_t flags, uint32_t stride = 1) {
+        uint64_t added = 0;
+        ctx_->check(rtq_order_model_log(model_.get(), bvh.scene_id(), rays_dev, n_rays, flags, stride, &added));
+        return added;
+    }
+    // mean loss of each epoch
+    std::vector<float> train(const rtq_train_params& tp) {
+        std::vector<float> loss(tp.epochs);
+        ctx_->check(rtq_order_model_train(model_.get(), &tp, loss.data()));
+        return loss;
+    }
+    void apply(const BVH& bvh, float confidence_closest = 1.0f, float confidence_any = 0.5f) {
+        ctx_->check(rtq_order_model_apply(model_.get(), bvh.scene_id(), confidence_closest, confidence_any));
+    }
+    void save(const std::string& path) { ctx_->check(rtq_order_model_save(model_.get(), path.c_str())); }
+    void load(const std::string& path) { ctx_->check(rtq_order_model_load(model_.get(), path.c_str())); }
+    void clear_samples() { ctx_->check(rtq_order_model_clear_samples(model_.get())); }
+    uint64_t samples() const {
+        uint64_t n = 0;
+        ctx_->check(rtq_order_model_info(model_.get(), &n, nullptr));
+        return n;
+    }
+    uint32_t steps() const {
+        uint32_t s = 0;
+        ctx_->check(rtq_order_model_info(model_.get(), nullptr, &s));
+        return s;
+    }
+
+  private:
+    struct Del {
+        void operator()(rtq_order_model* m) const { rtq_order_model_destroy(m); }
+    };
+    std::shared_ptr<Context> ctx_;   // declared first: destroyed after the model
+    std::unique_ptr<rtq_order_model, Del> model_;
 };
 
 }  // namespace rtq

@@ -369,6 +369,27 @@ rtq_status rtq_order_save(rtq_ctx* ctx, const char* path, const float* params_de
 rtq_status rtq_order_load(rtq_ctx* ctx, const char* path, float* params_dev, float* adam_m_dev,
                           float* adam_v_dev, uint32_t* step);
 
+/* The same as one object that owns its device memory (parameters, Adam
+ * moments and step, and a store of up to sample_capacity samples), so a caller
+ * needs no device allocation of its own.  Destroy a model before its context;
+ * calls on one model must not overlap. */
+typedef struct rtq_order_model rtq_order_model;
+rtq_status rtq_order_model_create(rtq_ctx* ctx, uint64_t seed, uint64_t sample_capacity, rtq_order_model** out);
+void rtq_order_model_destroy(rtq_order_model* model);
+/* rtq_order_log into the model's store, after the samples already there; a
+ * full store keeps what fits.  *n_added (may be NULL) receives the count. */
+rtq_status rtq_order_model_log(rtq_order_model* model, uint32_t scene_id, const rtq_ray* rays_dev,
+                               uint64_t n_rays, uint32_t flags, uint32_t stride, uint64_t* n_added);
+rtq_status rtq_order_model_clear_samples(rtq_order_model* model);
+/* rtq_order_train on the stored samples (epoch_loss may be NULL). */
+rtq_status rtq_order_model_train(rtq_order_model* model, const rtq_train_params* tp, float* epoch_loss);
+rtq_status rtq_order_model_apply(rtq_order_model* model, uint32_t scene_id, float confidence_closest,
+                                 float confidence_any);
+rtq_status rtq_order_model_save(rtq_order_model* model, const char* path);
+rtq_status rtq_order_model_load(rtq_order_model* model, const char* path);
+/* Stored samples and Adam steps taken (either pointer may be NULL). */
+rtq_status rtq_order_model_info(const rtq_order_model* model, uint64_t* n_samples, uint32_t* step);
+
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop
 #endif

@@ -10,5 +10,5 @@ from .rtq import (  # noqa: F401
     RTQ_BUILD_REF_PADDED_BOUNDS, RTQ_TRACE_ANY, RTQ_TRACE_BINARY, RTQ_TRACE_CLOSEST, RTQ_TRACE_LEAF_REFMT,
     RTQ_TRACE_LEARNED, ORDER_INPUTS, ORDER_HIDDEN, ORDER_PARAMS, ORDER_SAMPLE,
     MAX_BATCH_ITEMS, Batch, BatchItem, BuildParams, BvhInfo, Context, LaunchParams, Pinhole, RtqError, Scene,
-    TrainParams, TuneResult, load_library,
+    OrderModelHandle, TrainParams, TuneResult, load_library,
 )

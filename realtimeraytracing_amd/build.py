@@ -91,8 +91,11 @@ def build_cpp_check(verbose: bool = False) -> Path:
     out_dir = ROOT / "tests" / "cpp" / "build"
     out_dir.mkdir(exist_ok=True)
     exe = out_dir / "wrapper_check"
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
-           f"-L{PKG}", "-lrtq", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../../realtimeraytracing_amd"]
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}",
+           f"-I{rocm / 'include'}", str(src), "-o", str(exe),
+           f"-L{PKG}", "-lrtq", f"-L{rocm / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{PKG}",
+           "-Wl,-rpath,$ORIGIN/../../../realtimeraytracing_amd", f"-Wl,-rpath,{rocm / 'lib'}"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"wrapper_check build failed:\n{res.stdout}\n{res.stderr}")

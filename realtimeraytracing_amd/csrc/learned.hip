@@ -862,3 +862,116 @@ rtq_status rtq_order_load(rtq_ctx* ctx, const char* path, float* params_dev, flo
 }
 
 }  // extern "C"
+
+// ---- the order model as one object owning its device memory ----------------
+
+struct rtq_order_model {
+    rtq_ctx* ctx = nullptr;
+    float* params = nullptr;   // kNP
+    float* m = nullptr;        // kNP (Adam first moment)
+    float* v = nullptr;        // kNP (Adam second moment)
+    float* samples = nullptr;  // capacity x kSample
+    uint64_t capacity = 0;
+    uint64_t n_samples = 0;
+    uint32_t step = 0;
+};
+
+namespace {
+void free_model(rtq_order_model* m) {
+    if (m->params) (void)hipFree(m->params);
+    if (m->m) (void)hipFree(m->m);
+    if (m->v) (void)hipFree(m->v);
+    if (m->samples) (void)hipFree(m->samples);
+    delete m;
+}
+}  // namespace
+
+extern "C" {
+
+rtq_status rtq_order_model_create(rtq_ctx* ctx, uint64_t seed, uint64_t sample_capacity, rtq_order_model** out) {
+    if (!ctx) return RTQ_EINVAL;
+    if (!out) return fail(ctx, RTQ_EINVAL, "order_model_create: null output");
+    *out = nullptr;
+    if (sample_capacity > 0xFFFFFFFFull)
+        return fail(ctx, RTQ_ECAPACITY, "order_model_create: at most 2^32 - 1 samples");
+    RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    rtq_order_model* m = new rtq_order_model;
+    m->ctx = ctx;
+    m->capacity = sample_capacity;
+    hipError_t e = dev_alloc(&m->params, kNP);
+    if (e == hipSuccess) e = dev_alloc(&m->m, kNP);
+    if (e == hipSuccess) e = dev_alloc(&m->v, kNP);
+    if (e == hipSuccess) e = dev_alloc(&m->samples, (size_t)sample_capacity * kSample);
+    if (e == hipSuccess) e = hipMemset(m->m, 0, sizeof(float) * kNP);
+    if (e == hipSuccess) e = hipMemset(m->v, 0, sizeof(float) * kNP);
+    if (e != hipSuccess) {
+        free_model(m);
+        return hip_fail(ctx, e, "order_model_create");
+    }
+    const rtq_status s = rtq_order_init(ctx, seed, m->params);
+    if (s != RTQ_OK) {
+        free_model(m);
+        return s;
+    }
+    *out = m;
+    return RTQ_OK;
+}
+
+void rtq_order_model_destroy(rtq_order_model* model) {
+    if (!model) return;
+    (void)hipSetDevice(model->ctx->device);
+    free_model(model);
+}
+
+rtq_status rtq_order_model_log(rtq_order_model* model, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays,
+                               uint32_t flags, uint32_t stride, uint64_t* n_added) {
+    if (!model) return RTQ_EINVAL;
+    uint64_t k = 0;
+    const uint64_t room = model->capacity - model->n_samples;
+    if (room > 0) {
+        const rtq_status s = rtq_order_log(model->ctx, scene_id, rays_dev, n_rays, flags, stride,
+                                           model->samples + model->n_samples * kSample, room, &k);
+        if (s != RTQ_OK) return s;
+    }
+    model->n_samples += k;
+    if (n_added) *n_added = k;
+    return RTQ_OK;
+}
+
+rtq_status rtq_order_model_clear_samples(rtq_order_model* model) {
+    if (!model) return RTQ_EINVAL;
+    model->n_samples = 0;
+    return RTQ_OK;
+}
+
+rtq_status rtq_order_model_train(rtq_order_model* model, const rtq_train_params* tp, float* epoch_loss) {
+    if (!model) return RTQ_EINVAL;
+    if (model->n_samples == 0) return fail(model->ctx, RTQ_EINVAL, "order_model_train: no samples logged");
+    return rtq_order_train(model->ctx, model->samples, model->n_samples, tp, model->params, model->m, model->v,
+                           &model->step, epoch_loss);
+}
+
+rtq_status rtq_order_model_apply(rtq_order_model* model, uint32_t scene_id, float confidence_closest,
+                                 float confidence_any) {
+    if (!model) return RTQ_EINVAL;
+    return rtq_order_apply(model->ctx, scene_id, model->params, confidence_closest, confidence_any);
+}
+
+rtq_status rtq_order_model_save(rtq_order_model* model, const char* path) {
+    if (!model) return RTQ_EINVAL;
+    return rtq_order_save(model->ctx, path, model->params, model->m, model->v, model->step);
+}
+
+rtq_status rtq_order_model_load(rtq_order_model* model, const char* path) {
+    if (!model) return RTQ_EINVAL;
+    return rtq_order_load(model->ctx, path, model->params, model->m, model->v, &model->step);
+}
+
+rtq_status rtq_order_model_info(const rtq_order_model* model, uint64_t* n_samples, uint32_t* step) {
+    if (!model) return RTQ_EINVAL;
+    if (n_samples) *n_samples = model->n_samples;
+    if (step) *step = model->step;
+    return RTQ_OK;
+}
+
+}  // extern "C"

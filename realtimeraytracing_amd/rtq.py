@@ -51,6 +51,9 @@ EXPORTED = [
     "rtq_launch_params_get", "rtq_launch_params_set", "rtq_autotune", "rtq_tune_save", "rtq_tune_load",
     "rtq_bvh_export_trav", "rtq_order_log", "rtq_order_init", "rtq_order_grad", "rtq_order_train",
     "rtq_order_predict", "rtq_order_apply", "rtq_order_save", "rtq_order_load",
+    "rtq_order_model_create", "rtq_order_model_destroy", "rtq_order_model_log", "rtq_order_model_clear_samples",
+    "rtq_order_model_train", "rtq_order_model_apply", "rtq_order_model_save", "rtq_order_model_load",
+    "rtq_order_model_info",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -156,6 +159,15 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_order_apply": (st, [vp, u32, vp, ctypes.c_float, ctypes.c_float]),
         "rtq_order_save": (st, [vp, ctypes.c_char_p, vp, vp, vp, u32]),
         "rtq_order_load": (st, [vp, ctypes.c_char_p, vp, vp, vp, ctypes.POINTER(u32)]),
+        "rtq_order_model_create": (st, [vp, u64, u64, ctypes.POINTER(vp)]),
+        "rtq_order_model_destroy": (None, [vp]),
+        "rtq_order_model_log": (st, [vp, u32, vp, u64, u32, u32, ctypes.POINTER(u64)]),
+        "rtq_order_model_clear_samples": (st, [vp]),
+        "rtq_order_model_train": (st, [vp, ctypes.POINTER(TrainParams), vp]),
+        "rtq_order_model_apply": (st, [vp, u32, ctypes.c_float, ctypes.c_float]),
+        "rtq_order_model_save": (st, [vp, ctypes.c_char_p]),
+        "rtq_order_model_load": (st, [vp, ctypes.c_char_p]),
+        "rtq_order_model_info": (st, [vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -285,6 +297,50 @@ class Context:
 
     def tune_load(self, path) -> None:
         self.check(self.lib.rtq_tune_load(self.h, str(path).encode()))
+
+
+class OrderModelHandle:
+    """rtq_order_model: the order model owning its device memory (params,
+    Adam state, sample store)."""
+
+    def __init__(self, ctx: "Context", seed: int = 1, sample_capacity: int = 1 << 22):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.rtq_order_model_create(ctx.h, seed, sample_capacity, ctypes.byref(h)))
+        self.h = h
+
+    def log(self, scene: "Scene", rays_ptr: int, n: int, flags: int = 0, stride: int = 1) -> int:
+        k = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.rtq_order_model_log(self.h, scene.id, rays_ptr, n, flags, stride, ctypes.byref(k)))
+        return k.value
+
+    def clear_samples(self) -> None:
+        self.ctx.check(self.ctx.lib.rtq_order_model_clear_samples(self.h))
+
+    def train(self, tp: "TrainParams") -> np.ndarray:
+        losses = np.zeros(max(tp.epochs, 1), np.float32)
+        self.ctx.check(self.ctx.lib.rtq_order_model_train(self.h, ctypes.byref(tp), _ptr(losses)))
+        return losses[:tp.epochs]
+
+    def apply(self, scene: "Scene", confidence_closest: float = 1.0, confidence_any: float = 0.5) -> None:
+        self.ctx.check(self.ctx.lib.rtq_order_model_apply(self.h, scene.id, ctypes.c_float(confidence_closest),
+                                                          ctypes.c_float(confidence_any)))
+
+    def save(self, path) -> None:
+        self.ctx.check(self.ctx.lib.rtq_order_model_save(self.h, str(path).encode()))
+
+    def load(self, path) -> None:
+        self.ctx.check(self.ctx.lib.rtq_order_model_load(self.h, str(path).encode()))
+
+    def info(self) -> tuple[int, int]:
+        n, st = ctypes.c_uint64(), ctypes.c_uint32()
+        self.ctx.check(self.ctx.lib.rtq_order_model_info(self.h, ctypes.byref(n), ctypes.byref(st)))
+        return n.value, st.value
+
+    def destroy(self) -> None:
+        if self.h:
+            self.ctx.lib.rtq_order_model_destroy(self.h)
+            self.h = None
 
 
 class Batch:

@@ -1,10 +1,14 @@
 // Drives rtq::BVH (include/rtq/bvh.hpp) the way the reference's Scene drives
 // cr::BVH (scene.cpp:146-171): build from a triangle vector and a model vector,
-// read _InternalStruct, flatten to pre-order, trace.  Prints "wrapper_check OK"
-// and exits 0 when every check holds.
+// read _InternalStruct, flatten to pre-order, trace.  Then the frame-batch,
+// autotuner and order-model wrappers on device buffers (HIP runtime API).
+// Prints "wrapper_check OK" and exits 0 when every check holds.
+#include <hip/hip_runtime_api.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "rtq/bvh.hpp"
@@ -86,6 +90,100 @@ int main() {
         threw = (e.status == RTQ_EINVAL);
     }
     CHECK(threw);
+
+    // ---- frame batch + autotuner + learned order on device buffers --------
+    // a second scene: the grid plus a cloud of small occluders above it
+    std::vector<TriangleGPU> occ_tris = tris;
+    uint64_t st = 12345;
+    auto u01 = [&]() { st = st * 6364136223846793005ull + 1442695040888963407ull; return (st >> 40) / 16777216.0f; };
+    for (int k = 0; k < 4000; ++k) {
+        const float cx = u01(), cy = 0.05f + 0.5f * u01(), cz = u01();
+        occ_tris.push_back({Vec4{cx - 0.01f, cy, cz - 0.01f, 1.0f}, Vec4{cx + 0.01f, cy, cz, 1.0f},
+                            Vec4{cx, cy + 0.01f, cz + 0.01f, 1.0f}, 0});
+    }
+    rtq::BVH occ_bvh(ctx, static_cast<uint32_t>(occ_tris.size()), occ_tris, models);
+    // shadow rays from the grid towards a light above it, and closest-hit rays
+    std::vector<rtq_ray> shadow, closest;
+    for (int k = 0; k < 20000; ++k) {
+        rtq_ray r{};
+        r.o[0] = u01(); r.o[1] = 1e-4f; r.o[2] = u01();
+        r.d[0] = 0.5f - r.o[0]; r.d[1] = 2.0f; r.d[2] = 0.5f - r.o[2];
+        r.tmax = 1.0f;
+        shadow.push_back(r);
+        rtq_ray c = r;
+        c.o[1] = 1.5f; c.d[0] = 0.0f; c.d[1] = -1.0f; c.d[2] = 0.0f; c.tmax = INFINITY;
+        closest.push_back(c);
+    }
+    const size_t n_s = shadow.size(), n_c = closest.size();
+    rtq_ray *d_shadow = nullptr, *d_closest = nullptr;
+    rtq_hit *d_hs = nullptr, *d_hc = nullptr, *d_hs2 = nullptr, *d_hc2 = nullptr;
+    CHECK(hipMalloc(&d_shadow, n_s * sizeof(rtq_ray)) == hipSuccess);
+    CHECK(hipMalloc(&d_closest, n_c * sizeof(rtq_ray)) == hipSuccess);
+    CHECK(hipMalloc(&d_hs, n_s * sizeof(rtq_hit)) == hipSuccess && hipMalloc(&d_hs2, n_s * sizeof(rtq_hit)) == hipSuccess);
+    CHECK(hipMalloc(&d_hc, n_c * sizeof(rtq_hit)) == hipSuccess && hipMalloc(&d_hc2, n_c * sizeof(rtq_hit)) == hipSuccess);
+    CHECK(hipMemcpy(d_shadow, shadow.data(), n_s * sizeof(rtq_ray), hipMemcpyHostToDevice) == hipSuccess);
+    CHECK(hipMemcpy(d_closest, closest.data(), n_c * sizeof(rtq_ray), hipMemcpyHostToDevice) == hipSuccess);
+    auto hits_of = [&](const rtq_hit* d, size_t n) {
+        std::vector<rtq_hit> h(n);
+        (void)hipMemcpy(h.data(), d, n * sizeof(rtq_hit), hipMemcpyDeviceToHost);
+        return h;
+    };
+    // reference results: one trace per set
+    ctx->check(rtq_trace(ctx->get(), occ_bvh.scene_id(), d_shadow, n_s, RTQ_TRACE_ANY, d_hs, nullptr));
+    ctx->check(rtq_trace(ctx->get(), occ_bvh.scene_id(), d_closest, n_c, RTQ_TRACE_CLOSEST, d_hc, nullptr));
+    ctx->check(rtq_synchronize(ctx->get()));
+    const std::vector<rtq_hit> ref_s = hits_of(d_hs, n_s), ref_c = hits_of(d_hc, n_c);
+    size_t occluded = 0;
+    for (const rtq_hit& h : ref_s) occluded += h.id >= 0;
+    CHECK(occluded > n_s / 10 && occluded < n_s);
+    // both sets in one graph-replayed launch with autotuned settings
+    std::vector<rtq_batch_item> items = {
+        {occ_bvh.scene_id(), RTQ_TRACE_ANY, d_shadow, d_hs2, n_s},
+        {occ_bvh.scene_id(), RTQ_TRACE_CLOSEST, d_closest, d_hc2, n_c},
+    };
+    const rtq_tune_result tuned = rtq::autotune(*ctx, items, 1);
+    CHECK(tuned.n_tried > 1 && tuned.best_ms <= tuned.default_ms);
+    {
+        rtq::FrameBatch frame(ctx, items);
+        for (int rep = 0; rep < 3; ++rep) {
+            CHECK(hipMemset(d_hc2, 0, n_c * sizeof(rtq_hit)) == hipSuccess);
+            frame.launch();
+            ctx->check(rtq_synchronize(ctx->get()));
+            const std::vector<rtq_hit> got_c = hits_of(d_hc2, n_c), got_s = hits_of(d_hs2, n_s);
+            CHECK(std::memcmp(got_c.data(), ref_c.data(), n_c * sizeof(rtq_hit)) == 0);
+            for (size_t i = 0; i < n_s; ++i) CHECK((got_s[i].id >= 0) == (ref_s[i].id >= 0));
+        }
+    }
+    // learned order: log, train, apply, trace with RTQ_TRACE_LEARNED, checkpoint
+    rtq::OrderModel model(ctx, 7, 1u << 20);
+    const uint64_t logged = model.log(occ_bvh, d_shadow, n_s, RTQ_TRACE_ANY, 1);
+    CHECK(logged > 0 && model.samples() == logged);
+    const rtq_train_params tp{3, 1024, 2e-3f, 0.9f, 0.999f, 0, 7};
+    const std::vector<float> loss = model.train(tp);
+    CHECK(loss.size() == 3 && loss[2] < loss[0] && model.steps() > 0);
+    model.apply(occ_bvh, 1.0f, 0.5f);
+    ctx->check(rtq_trace(ctx->get(), occ_bvh.scene_id(), d_shadow, n_s, RTQ_TRACE_ANY | RTQ_TRACE_LEARNED, d_hs2,
+                         nullptr));
+    ctx->check(rtq_trace(ctx->get(), occ_bvh.scene_id(), d_closest, n_c, RTQ_TRACE_LEARNED, d_hc2, nullptr));
+    ctx->check(rtq_synchronize(ctx->get()));
+    {
+        const std::vector<rtq_hit> got_c = hits_of(d_hc2, n_c), got_s = hits_of(d_hs2, n_s);
+        CHECK(std::memcmp(got_c.data(), ref_c.data(), n_c * sizeof(rtq_hit)) == 0);
+        for (size_t i = 0; i < n_s; ++i) CHECK((got_s[i].id >= 0) == (ref_s[i].id >= 0));
+    }
+    const char* ckpt = "wrapper_check_order.bin";
+    model.save(ckpt);
+    rtq::OrderModel other(ctx, 99, 16);
+    other.load(ckpt);
+    CHECK(other.steps() == model.steps());
+    std::remove(ckpt);
+    (void)hipFree(d_shadow);
+    (void)hipFree(d_closest);
+    (void)hipFree(d_hs);
+    (void)hipFree(d_hs2);
+    (void)hipFree(d_hc);
+    (void)hipFree(d_hc2);
+
     const rtq_bvh_info info = bvh.info();
     std::printf("wrapper_check OK: %u triangles, %u traversal nodes, depth %u, build %.3f ms\n", info.n_tris,
                 info.n_trav_nodes, info.depth, info.build_ms);

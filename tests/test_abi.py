@@ -92,3 +92,19 @@ def test_argument_errors_without_device():
     assert lib.rtq_bvh_build(None, 1, None) == RTQ_EINVAL
     assert lib.rtq_trace(None, 1, None, 0, 0, None, None) == RTQ_EINVAL
     assert lib.rtq_create(None, 0) == RTQ_EINVAL
+    # the batch, tuning and learned-ordering entry points too
+    n = ctypes.c_uint64()
+    assert lib.rtq_bvh_export_trav(None, 1, None, None, 0, ctypes.byref(n)) == RTQ_EINVAL
+    assert lib.rtq_trace_batch(None, None, 0, None) == RTQ_EINVAL
+    assert lib.rtq_autotune(None, None, 0, 1, None) == RTQ_EINVAL
+    assert lib.rtq_order_log(None, 1, None, 0, 0, 1, None, 0, ctypes.byref(n)) == RTQ_EINVAL
+    assert lib.rtq_order_init(None, 1, None) == RTQ_EINVAL
+    assert lib.rtq_order_grad(None, None, None, 1, None, None) == RTQ_EINVAL
+    assert lib.rtq_order_train(None, None, 1, None, None, None, None, None, None) == RTQ_EINVAL
+    assert lib.rtq_order_apply(None, 1, None, 0.5, 0.5) == RTQ_EINVAL
+    assert lib.rtq_order_model_create(None, 1, 16, None) == RTQ_EINVAL
+    assert lib.rtq_order_model_log(None, 1, None, 0, 0, 1, ctypes.byref(n)) == RTQ_EINVAL
+    assert lib.rtq_order_model_train(None, None, None) == RTQ_EINVAL
+    assert lib.rtq_order_model_apply(None, 1, 1.0, 0.5) == RTQ_EINVAL
+    assert lib.rtq_order_model_info(None, ctypes.byref(n), None) == RTQ_EINVAL
+    lib.rtq_order_model_destroy(None)   # no-op

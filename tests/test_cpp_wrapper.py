@@ -1,5 +1,6 @@
-"""The C++ mirror of cr::BVH (include/rtq/bvh.hpp): compiles as C++17 here,
-and the prebuilt check program passes on the GPU."""
+"""The C++ mirror of cr::BVH and the batch / order-model wrappers
+(include/rtq/bvh.hpp): compiles as C++17 here, and the prebuilt check
+program passes on the GPU."""
 import shutil
 import subprocess
 from pathlib import Path
@@ -13,7 +14,8 @@ EXE = ROOT / "tests" / "cpp" / "build" / "wrapper_check"
 def test_wrapper_header_compiles(tmp_path):
     if shutil.which("g++") is None:
         pytest.skip("g++ missing")
-    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{ROOT / 'include'}",
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__",
+                    f"-I{ROOT / 'include'}", "-I/opt/rocm/include",
                     str(ROOT / "tests" / "cpp" / "wrapper_check.cpp")], check=True)
 
 

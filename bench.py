@@ -73,6 +73,7 @@ def parse():
     ap.add_argument("--occluded-res", type=int, default=2048, help="primary image side of the occluded workload")
     ap.add_argument("--no-compressed-8m", action="store_true", help="skip the 4-terrain 8-wide workload")
     ap.add_argument("--no-uneven-batch", action="store_true", help="skip the 64-scene uneven batch")
+    ap.add_argument("--no-shade", action="store_true", help="skip shading / BVH-view timing")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
@@ -349,6 +350,47 @@ def main() -> int:
                "sample": f"all {wl.n_diff} diffuse + {wl.n_shad} shadow rays of one step, reference traversal "
                          f"semantics (RefMT, no pruning), {cpu_s:.2f} s",
                "watertight_pruned_Mrays_s": round(wl.live_rays / cpu2_s / 1e6, 3)}
+
+    # caller side of the path: shading the primary hits into an RGBA32F image
+    # (getColor with wireframe) and the BVH view at depth 8, on the same frame
+    shading = None
+    if not args.no_shade:
+        from realtimeraytracing_amd import HIT_DTYPE, RTQ_SHADE_WIREFRAME
+        from realtimeraytracing_amd.render import materials_array
+        n = wl.n_diff
+        f4 = np.dtype((np.float32, 4))
+        mats = devmem.to_device(materials_array([[0.2, 0.3, 0.1, 1.0]]))
+        rgba, ov = devmem.empty(n, f4), devmem.empty(n, f4)
+        torch.cuda.synchronize()
+
+        def shade_step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            wl.sc.shade(wl.phit.data_ptr(), n, mats.data_ptr(), 1, rgba.data_ptr(), None, RTQ_SHADE_WIREFRAME, sp)
+            if ev is not None:
+                ev[1].record(stream)
+                ev[2].record(stream)
+
+        def overlay_step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            wl.sc.bvh_overlay(wl.prim.data_ptr(), n, 8, ov.data_ptr(), sp)
+            if ev is not None:
+                ev[1].record(stream)
+                ev[2].record(stream)
+        shade_ms, _ = time_steps(shade_step)
+        ov_ms, _ = time_steps(overlay_step)
+        n_hit = int((devmem.to_host(wl.phit, HIT_DTYPE)["id"] >= 0).sum())
+        # per pixel: 16-B hit in, 16-B pixel out; per hit: the triangle's model id,
+        # the model's material id and the 16-B colour (4 + 4 + 16 B)
+        sbytes = 32 * n + 24 * n_hit
+        shading = {"pixels": n, "hits": n_hit, "shade_ms": round(shade_ms, 5),
+                   "shade_GB_s": round(sbytes / (shade_ms * 1e-3) / 1e9, 1),
+                   "shade_hbm_frac": round(sbytes / (shade_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "shade_algorithmic_bytes": sbytes,
+                   "overlay_depth": 8, "overlay_ms": round(ov_ms, 4),
+                   "overlay_Mrays_s": round(n / (ov_ms * 1e-3) / 1e6, 1)}
+        del rgba, ov, mats
     wl.release()
 
     # learned-ordering workload: terrain + soup, plain against learned order
@@ -546,6 +588,7 @@ def main() -> int:
                    "occluded": occluded,
                    "compressed_8m": compressed_8m,
                    "uneven_batch": uneven,
+                   "shading": shading,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
                    "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
                    "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},

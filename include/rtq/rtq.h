@@ -71,6 +71,23 @@ rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_t
                             uint32_t* scene_id);
 rtq_status rtq_scene_release(rtq_ctx* ctx, uint32_t scene_id);
 
+/* ---- OBJ ingest (host only; no context or device needed) ------------------ */
+
+/* Mesh::load (mesh.cpp:186-263): the triangles of a Wavefront OBJ file in
+ * file order as TriangleGPU records (w = 1, model_id as given), the way the
+ * reference reads it through tinyobjloader with triangulation on.  Reads `v`
+ * and `f` statements (v, v/vt, v//vn, v/vt/vn references, 1-based or
+ * negative); faces with fewer than 3 vertices are skipped; a quad is split
+ * along its shorter diagonal (tinyobjloader 2.x), larger polygons into a fan
+ * (0, k, k+1).  Call with tris == NULL to get *n_tris, then with a buffer of
+ * `capacity` records (RTQ_ECAPACITY when too small).  Malformed input gives
+ * RTQ_EINVAL with a message in msg (may be NULL; msg_capacity bytes). */
+rtq_status rtq_obj_load(const char* path, uint32_t model_id, rtq_triangle* tris, uint64_t capacity,
+                        uint64_t* n_tris, char* msg, uint64_t msg_capacity);
+/* The same for OBJ text in memory (len bytes). */
+rtq_status rtq_obj_parse(const char* text, uint64_t len, uint32_t model_id, rtq_triangle* tris,
+                         uint64_t capacity, uint64_t* n_tris, char* msg, uint64_t msg_capacity);
+
 /* ---- BVH build ---------------------------------------------------------- */
 
 /* Build flags. */
@@ -292,6 +309,40 @@ rtq_status rtq_raygen_diffuse(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* pr
 rtq_status rtq_raygen_shadow(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* prim_dev,
                              const rtq_hit* hits_dev, uint64_t n, const float* lights,
                              uint32_t n_lights, float eps, rtq_ray* out_dev, void* stream);
+
+/* ---- shading into an image (caller side of the path) -------------------- */
+
+/* Shading flags. */
+#define RTQ_SHADE_WIREFRAME 0x1u   /* uIsWireframeModeOn (raytracer.glsl:170-178) */
+
+/* getColor + main's pixel value (raytracer.glsl:159-179, 299-331), one RGBA32F
+ * pixel per hit record:
+ *   c = (0, 0, 0, 1), or overlay[i] when overlay_dev is not NULL (the BVH view,
+ *       uIsBVHDisplayed: c = bvhColor);
+ *   a hit adds the colour of material models[tris[id].model_id].material_id
+ *       (nothing for an id >= n_materials, as a read of the zero-filled
+ *       material storage past the used entries);
+ *   with RTQ_SHADE_WIREFRAME a hit with a barycentric weight below 0.02 is
+ *       (0, 0, 0, 1).  The weights are (u, v, (1 - u) - v) of the hit record,
+ *       the shader's (b0, b1, b2) for RefMT hits.
+ * The reference's pixel keeps alpha 1 + material alpha (its += on the whole
+ * vec4); so does this.  Device buffers of n records / pixels. */
+rtq_status rtq_shade(rtq_ctx* ctx, uint32_t scene_id, const rtq_hit* hits_dev, uint64_t n,
+                     const rtq_material* materials_dev, uint32_t n_materials, const float* overlay_dev,
+                     uint32_t flags, float* rgba_dev, void* stream);
+
+/* bvhColor of getClosestHitBVH (raytracer.glsl:246-276) for each ray: the
+ * reference's DFS over the pre-order node array (left child pushed first, so
+ * the right child is visited first) with its slab test (intersectBVH,
+ * raytracer.glsl:182-237, GLSL min/max, 1/d per node, no ray interval); every
+ * node at `depth` that the ray reaches sets the colour, the last one wins:
+ * (0.7, 0, 0.7, 0.1) when the entry point lies within 0.05 / (depth + 1) of
+ * two of the box's slab planes, else (0.5, 0, 0.5, 0.1); (0, 0, 0, 0) when no
+ * node at `depth` is reached.  Nodes below `depth` cannot change the colour and
+ * are not visited.  rgba_dev: n float4.  A depth above 62 on a tree that deep
+ * returns RTQ_ECAPACITY. */
+rtq_status rtq_bvh_overlay(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n, uint32_t depth,
+                           float* rgba_dev, void* stream);
 
 /* ---- learned child ordering (no counterpart in the reference: its
  * traversal pushes left then right, so it always enters the right child

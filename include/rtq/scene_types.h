@@ -8,6 +8,7 @@
  *   rtq_mesh_model   cr::MeshModelGPU  mesh.hpp:12-15         (68 B, column-major mat4 + u32)
  *   rtq_aabb         cr::AABB_GPU      bvh.hpp:22-26          (32 B, _Min @0, _Max @16)
  *   rtq_bvh_node     cr::BVH_NodeGPU   bvh.hpp:36-42          (48 B, ids @32/36/40)
+ *   rtq_material     cr::MaterialGPU   material.hpp:9-11      (16 B, RGBA)
  *
  * Ray and hit records have no counterpart in the reference (its shader builds a
  * ray per pixel and shades straight into an image, raytracer.glsl:92-100,299-331);
@@ -62,6 +63,11 @@ typedef struct rtq_bvh_node {
     uint32_t pad_;
 } rtq_bvh_node;
 
+/* cr::MaterialGPU (material.hpp:9-11): one RGBA colour. */
+typedef struct rtq_material {
+    float color[4];
+} rtq_material;
+
 /* Ray: origin, interval start, direction (need not be unit length), interval end.
  * A hit is reported for tmin <= t <= tmax. */
 typedef struct rtq_ray {
@@ -95,6 +101,7 @@ static_assert(sizeof(rtq_bvh_node) == 48, "cr::BVH_NodeGPU is 48 B");
 static_assert(offsetof(rtq_bvh_node, triangle_id) == 32, "_TriangleId @32");
 static_assert(offsetof(rtq_bvh_node, left_child) == 36, "_LeftChild @36");
 static_assert(offsetof(rtq_bvh_node, right_child) == 40, "_RightChild @40");
+static_assert(sizeof(rtq_material) == 16, "cr::MaterialGPU is 16 B");
 static_assert(sizeof(rtq_ray) == 32, "ray record is 32 B");
 static_assert(sizeof(rtq_hit) == 16, "hit record is 16 B");
 #else
@@ -102,6 +109,7 @@ _Static_assert(sizeof(rtq_triangle) == 64, "cr::TriangleGPU is 64 B");
 _Static_assert(sizeof(rtq_mesh_model) == 68, "cr::MeshModelGPU is 68 B");
 _Static_assert(sizeof(rtq_aabb) == 32, "cr::AABB_GPU is 32 B");
 _Static_assert(sizeof(rtq_bvh_node) == 48, "cr::BVH_NodeGPU is 48 B");
+_Static_assert(sizeof(rtq_material) == 16, "cr::MaterialGPU is 16 B");
 _Static_assert(sizeof(rtq_ray) == 32, "ray record is 32 B");
 _Static_assert(sizeof(rtq_hit) == 16, "hit record is 16 B");
 #endif

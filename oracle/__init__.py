@@ -72,6 +72,9 @@ def lib() -> ctypes.CDLL:
             "orc_mlp_loss": (ctypes.c_double, [vp, vp, u64]),
             "orc_mlp_grad": (ctypes.c_double, [vp, vp, u64, vp]),
             "orc_order_features": (None, [vp, vp, u32, u32, vp]),
+            "orc_ref_box_hit": (i32, [vp, vp, f32]),
+            "orc_bvh_overlay": (None, [vp, u32, vp, u64, u32, i32, vp]),
+            "orc_shade": (None, [vp, u32, vp, vp, u32, vp, u64, vp, u32, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -290,3 +293,37 @@ def order_features(node16, aux4, octant, any_hit) -> np.ndarray:
     f = np.zeros(32, np.float32)
     lib().orc_order_features(_p(nd), _p(ax), int(octant), int(any_hit), _p(f))
     return f
+
+
+# ---- shading (shade.c) ----------------------------------------------------------
+
+def ref_box_hit(ray, box, threshold: float) -> int:
+    """intersectBVH (raytracer.glsl:182-237): 0 miss, 1 hit, 2 hit near two slab planes."""
+    TRI, MOD, AABB, NODE, RAY, HIT = _dtypes()
+    r = _c(ray, RAY)
+    b = _c(box, AABB)
+    return lib().orc_ref_box_hit(_p(r), _p(b), threshold)
+
+
+def bvh_overlay(nodes, rays, depth: int, cut: bool = False) -> np.ndarray:
+    """bvhColor per ray (raytracer.glsl:246-276), float32 [n, 4]."""
+    TRI, MOD, AABB, NODE, RAY, HIT = _dtypes()
+    nodes = _c(nodes, NODE)
+    rays = _c(rays, RAY)
+    out = np.zeros((len(rays), 4), np.float32)
+    lib().orc_bvh_overlay(_p(nodes), len(nodes), _p(rays), len(rays), depth, int(cut), _p(out))
+    return out
+
+
+def shade(tris, models, materials, hits, overlay=None, wireframe=False) -> np.ndarray:
+    """Pixel colour per hit record (raytracer.glsl:159-179, 299-331), float32 [n, 4]."""
+    TRI, MOD, AABB, NODE, RAY, HIT = _dtypes()
+    tris = _c(tris, TRI)
+    models = _c(models, MOD)
+    mats = _c(np.asarray(materials, np.float32).reshape(-1, 4))
+    hits = _c(hits, HIT)
+    ov = None if overlay is None else _c(np.asarray(overlay, np.float32).reshape(-1, 4))
+    out = np.zeros((len(hits), 4), np.float32)
+    lib().orc_shade(_p(tris), len(tris), _p(models), _p(mats), len(mats), _p(hits), len(hits), _p(ov),
+                    1 if wireframe else 0, _p(out))
+    return out

@@ -29,6 +29,14 @@
  *                                    orc_mlp_loss; all-zero network closed form
  *   orc_mlp_logits                   hand-built networks with known outputs
  *   orc_order_features               hand-computed features of a hand node
+ *   orc_ref_box_hit                  hand boxes: inside / behind / border /
+ *                                    corner cases with known answers
+ *   orc_bvh_overlay                  cut and uncut walks agree; a numpy
+ *                                    restatement (a depth-d node is reached iff
+ *                                    it and all its ancestors are hit; the
+ *                                    right-first DFS reaches the lowest
+ *                                    pre-order index last)
+ *   orc_shade                        hand hit records with known colours
  */
 #ifndef RTQ_ORACLE_H
 #define RTQ_ORACLE_H
@@ -160,6 +168,18 @@ double orc_mlp_loss(const float* params, const float* samples, uint64_t n);
 double orc_mlp_grad(const float* params, const float* samples, uint64_t n, double* grad);
 /* Features of one 64-B traversal node for (octant, mode); see order.c. */
 void orc_order_features(const float* node16, const uint32_t* aux4, uint32_t octant, uint32_t any, float* f32);
+
+/* ---- shading (shade.c): raytracer.glsl:159-179, 182-276, 299-331 -------- */
+/* intersectBVH: 0 miss, 1 hit, 2 hit near two slab planes of the box. */
+int orc_ref_box_hit(const rtq_ray* ray, const rtq_aabb* b, float threshold);
+/* bvhColor per ray for uDepthDisplayBVH = depth (4 floats per ray); cut != 0
+ * stops descending below the display depth. */
+void orc_bvh_overlay(const rtq_bvh_node* nodes, uint32_t n_nodes, const rtq_ray* rays, uint64_t n,
+                     uint32_t depth, int cut, float* rgba);
+/* Pixel colour per hit record (flags bit 0 = wireframe); overlay may be NULL. */
+void orc_shade(const rtq_triangle* tris, uint32_t n_tris, const rtq_mesh_model* models, const rtq_material* materials,
+               uint32_t n_materials, const rtq_hit* hits, uint64_t n, const float* overlay, uint32_t flags,
+               float* rgba);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "librtq.so"
-SOURCES = ["api.hip", "batch.hip", "build.hip", "learned.hip", "primitives.hip", "trace.hip", "wide.hip"]
+SOURCES = ["api.hip", "batch.hip", "build.hip", "learned.hip", "obj.cpp", "primitives.hip", "shade.hip",
+           "trace.hip", "wide.hip"]
 ARCH = os.environ.get("RTQ_OFFLOAD_ARCH", "gfx950")
 
 
@@ -50,9 +51,15 @@ def compile_flags() -> list[str]:
     ]
 
 
+def host_flags() -> list[str]:
+    """Host-only sources (.cpp): same language level and FP rules, no device code."""
+    return ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-ffp-contract=off", "-Wall", f"-I{ROOT / 'include'}"]
+
+
 def _compile(src: str) -> Path:
     obj = OBJ / (Path(src).stem + ".o")
-    cmd = [hipcc(), *compile_flags(), "-c", str(CSRC / src), "-o", str(obj)]
+    flags = host_flags() if src.endswith(".cpp") else compile_flags()
+    cmd = [hipcc(), *flags, "-c", str(CSRC / src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stdout}\n{res.stderr}")

@@ -32,9 +32,19 @@ def zeros(n: int, dtype: np.dtype, device: str = "cuda"):
 
 
 def to_host(t, dtype: np.dtype) -> np.ndarray:
-    """Device uint8 tensor -> numpy array of `dtype` (synchronous copy)."""
-    return t.cpu().numpy().view(dtype).copy()
+    """Device uint8 tensor -> numpy array of `dtype` (synchronous copy); a
+    subarray dtype such as (float32, 4) gives shape (n, 4)."""
+    dtype = np.dtype(dtype)
+    raw = t.cpu().numpy()
+    if dtype.shape:
+        return raw.view(dtype.base).reshape(-1, *dtype.shape).copy()
+    return raw.view(dtype).copy()
 
 
 def ptr(t) -> int:
     return t.data_ptr()
+
+
+def synchronize() -> None:
+    """Wait for torch's work on the current device (copies into buffers the library reads)."""
+    _torch().cuda.synchronize()

@@ -39,6 +39,8 @@ RTQ_TRACE_ANY = 0x1
 RTQ_TRACE_LEAF_REFMT = 0x2
 RTQ_TRACE_BINARY = 0x4
 RTQ_TRACE_LEARNED = 0x8
+RTQ_SHADE_WIREFRAME = 0x1
+MATERIAL_DTYPE = np.dtype({"names": ["color"], "formats": [("<f4", 4)], "offsets": [0], "itemsize": 16})
 ORDER_INPUTS, ORDER_HIDDEN, ORDER_PARAMS, ORDER_SAMPLE = 32, 64, 6337, 34
 
 EXPORTED = [
@@ -53,7 +55,7 @@ EXPORTED = [
     "rtq_order_predict", "rtq_order_apply", "rtq_order_save", "rtq_order_load",
     "rtq_order_model_create", "rtq_order_model_destroy", "rtq_order_model_log", "rtq_order_model_clear_samples",
     "rtq_order_model_train", "rtq_order_model_apply", "rtq_order_model_save", "rtq_order_model_load",
-    "rtq_order_model_info",
+    "rtq_order_model_info", "rtq_shade", "rtq_bvh_overlay", "rtq_obj_load", "rtq_obj_parse",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -168,6 +170,10 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_order_model_save": (st, [vp, ctypes.c_char_p]),
         "rtq_order_model_load": (st, [vp, ctypes.c_char_p]),
         "rtq_order_model_info": (st, [vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
+        "rtq_shade": (st, [vp, u32, vp, u64, vp, u32, vp, u32, vp, vp]),
+        "rtq_obj_load": (st, [ctypes.c_char_p, u32, vp, u64, ctypes.POINTER(u64), ctypes.c_char_p, u64]),
+        "rtq_obj_parse": (st, [ctypes.c_char_p, u64, u32, vp, u64, ctypes.POINTER(u64), ctypes.c_char_p, u64]),
+        "rtq_bvh_overlay": (st, [vp, u32, vp, u64, u32, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -180,6 +186,30 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
 
 def _ptr(a: np.ndarray | None) -> int | None:
     return None if a is None else a.ctypes.data
+
+
+def _obj_call(fn, head: tuple, model_id: int) -> np.ndarray:
+    n = ctypes.c_uint64()
+    msg = ctypes.create_string_buffer(256)
+    rc = fn(*head, model_id, None, 0, ctypes.byref(n), msg, len(msg))
+    if rc != RTQ_OK:
+        raise RtqError(rc, msg.value.decode(errors="replace"))
+    out = np.zeros(n.value, TRIANGLE_DTYPE)
+    rc = fn(*head, model_id, _ptr(out), n.value, ctypes.byref(n), msg, len(msg))
+    if rc != RTQ_OK:
+        raise RtqError(rc, msg.value.decode(errors="replace"))
+    return out
+
+
+def obj_load(path, model_id: int = 0) -> np.ndarray:
+    """Triangles of a Wavefront OBJ file (rtq_obj_load: Mesh::load, mesh.cpp:186-263)."""
+    return _obj_call(load_library().rtq_obj_load, (str(path).encode(),), model_id)
+
+
+def obj_parse(text, model_id: int = 0) -> np.ndarray:
+    """Triangles of OBJ text in memory (rtq_obj_parse)."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    return _obj_call(load_library().rtq_obj_parse, (data, len(data)), model_id)
 
 
 class Context:
@@ -446,6 +476,16 @@ class Scene:
                        stream: int | None = None) -> None:
         self.ctx.check(self.ctx.lib.rtq_raygen_diffuse(self.ctx.h, self.id, prim_ptr, hits_ptr, n, seed,
                                                        ctypes.c_float(eps), out_ptr, stream))
+
+    def shade(self, hits_ptr: int, n: int, materials_ptr: int | None, n_materials: int, rgba_ptr: int,
+              overlay_ptr: int | None = None, flags: int = 0, stream: int | None = None) -> None:
+        """RGBA32F pixel per hit record (device buffers; materials: n_materials x 16 B)."""
+        self.ctx.check(self.ctx.lib.rtq_shade(self.ctx.h, self.id, hits_ptr, n, materials_ptr, n_materials,
+                                              overlay_ptr, flags, rgba_ptr, stream))
+
+    def bvh_overlay(self, rays_ptr: int, n: int, depth: int, rgba_ptr: int, stream: int | None = None) -> None:
+        """Colour of the reference's BVH view at `depth` per ray (device buffers)."""
+        self.ctx.check(self.ctx.lib.rtq_bvh_overlay(self.ctx.h, self.id, rays_ptr, n, depth, rgba_ptr, stream))
 
     def raygen_shadow(self, prim_ptr: int, hits_ptr: int, n: int, lights: np.ndarray, eps: float, out_ptr: int,
                       stream: int | None = None) -> None:

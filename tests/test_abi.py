@@ -13,7 +13,7 @@ import pytest
 from realtimeraytracing_amd import (AABB_DTYPE, EXPORTED, HIT_DTYPE, MODEL_DTYPE, NODE_DTYPE, RAY_DTYPE,
                                     TRIANGLE_DTYPE, BatchItem, BuildParams, BvhInfo, LaunchParams, Pinhole,
                                     TrainParams, TuneResult, load_library)
-from realtimeraytracing_amd.rtq import RTQ_EINVAL
+from realtimeraytracing_amd.rtq import MATERIAL_DTYPE, RTQ_EINVAL
 
 ROOT = Path(__file__).resolve().parents[1]
 HEADERS = sorted((ROOT / "include" / "rtq").glob("*.h"))
@@ -47,14 +47,15 @@ def test_numpy_layouts():
     assert NODE_DTYPE.itemsize == 48
     assert [NODE_DTYPE.fields[k][1] for k in ("triangle_id", "left_child", "right_child")] == [32, 36, 40]
     assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+    assert MATERIAL_DTYPE.itemsize == 16
 
 
 SIZES_C = r"""
 #include <stdio.h>
 #include "rtq/rtq.h"
 int main(void) {
-    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rtq_triangle), sizeof(rtq_mesh_model),
-           sizeof(rtq_aabb), sizeof(rtq_bvh_node), sizeof(rtq_ray), sizeof(rtq_hit),
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rtq_triangle), sizeof(rtq_mesh_model),
+           sizeof(rtq_aabb), sizeof(rtq_bvh_node), sizeof(rtq_ray), sizeof(rtq_hit), sizeof(rtq_material),
            sizeof(rtq_build_params), sizeof(rtq_bvh_info), sizeof(rtq_pinhole), sizeof(rtq_batch_item),
            sizeof(rtq_launch_params), sizeof(rtq_tune_result), sizeof(rtq_train_params));
     return 0;
@@ -72,7 +73,7 @@ def test_headers_compile_and_sizes(tmp_path, compiler, std):
     subprocess.run([compiler, std, "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
                    check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
-    assert sizes == [64, 68, 32, 48, 32, 16] + [ctypes.sizeof(c) for c in (
+    assert sizes == [64, 68, 32, 48, 32, 16, 16] + [ctypes.sizeof(c) for c in (
         BuildParams, BvhInfo, Pinhole, BatchItem, LaunchParams, TuneResult, TrainParams)]
 
 
@@ -108,3 +109,5 @@ def test_argument_errors_without_device():
     assert lib.rtq_order_model_apply(None, 1, 1.0, 0.5) == RTQ_EINVAL
     assert lib.rtq_order_model_info(None, ctypes.byref(n), None) == RTQ_EINVAL
     lib.rtq_order_model_destroy(None)   # no-op
+    assert lib.rtq_shade(None, 1, None, 0, None, 0, None, 0, None, None) == RTQ_EINVAL
+    assert lib.rtq_bvh_overlay(None, 1, None, 0, 0, None, None) == RTQ_EINVAL

@@ -74,6 +74,7 @@ def parse():
     ap.add_argument("--no-compressed-8m", action="store_true", help="skip the 4-terrain 8-wide workload")
     ap.add_argument("--no-uneven-batch", action="store_true", help="skip the 64-scene uneven batch")
     ap.add_argument("--no-shade", action="store_true", help="skip shading / BVH-view timing")
+    ap.add_argument("--no-refit", action="store_true", help="skip the scene-edit refit timing")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
@@ -391,6 +392,36 @@ def main() -> int:
                    "overlay_depth": 8, "overlay_ms": round(ov_ms, 4),
                    "overlay_Mrays_s": round(n / (ov_ms * 1e-3) / 1e6, 1)}
         del rgba, ov, mats
+
+    # scene edit: every terrain vertex moved (heights rippled by up to 0.02),
+    # then a refit of the kept tree against a full rebuild
+    refit = None
+    if not args.no_refit:
+        tris2 = tris.copy()
+        for name in ("p0", "p1", "p2"):
+            x = tris2[name][:, 0].astype(np.float64)
+            z = tris2[name][:, 2].astype(np.float64)
+            tris2[name][:, 1] = (tris2[name][:, 1] + 0.02 * np.sin(12.0 * x) * np.cos(9.0 * z)).astype(np.float32)
+        torch.cuda.synchronize()
+        t_u = time.perf_counter()
+        wl.sc.update(tris2)
+        update_ms = (time.perf_counter() - t_u) * 1e3
+        refit_ms = []
+        for _ in range(3):
+            t_r = time.perf_counter()
+            wl.sc.refit()
+            refit_ms.append((time.perf_counter() - t_r) * 1e3)
+        ms_refit, _ = time_steps(two_launches(wl, RTQ_TRACE_BINARY))
+        h_refit = wl.hits()
+        rinfo = wl.sc.build(16, args.max_leaf, 0, node_width=8)
+        ms_rebuild, _ = time_steps(two_launches(wl, RTQ_TRACE_BINARY))
+        h_rebuild = wl.hits()
+        refit = {"workload": "terrain vertices moved (y += 0.02 sin(12x) cos(9z)), plain binary diffuse + shadow",
+                 "update_ms": round(update_ms, 2), "refit_ms": round(min(refit_ms), 3),
+                 "rebuild_ms": round(rinfo.build_ms, 3),
+                 "Mrays_s_after_refit": round(wl.live_rays / (ms_refit * 1e-3) / 1e6, 1),
+                 "Mrays_s_after_rebuild": round(wl.live_rays / (ms_rebuild * 1e-3) / 1e6, 1),
+                 "exact_refit_vs_rebuild": same_results(h_rebuild, h_refit)}
     wl.release()
 
     # learned-ordering workload: terrain + soup, plain against learned order
@@ -589,6 +620,7 @@ def main() -> int:
                    "compressed_8m": compressed_8m,
                    "uneven_batch": uneven,
                    "shading": shading,
+                   "refit": refit,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
                    "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
                    "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},

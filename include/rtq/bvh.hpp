@@ -113,6 +113,29 @@ class BVH {
         return hits;
     }
 
+    // Per-frame edit: new vertices and/or model matrices (same counts; an empty
+    // vector keeps that array), then new boxes for the kept tree.  The
+    // _InternalStruct boxes are refreshed.
+    template <class Tri, class Model>
+    void update(const std::vector<Tri>& triangles, const std::vector<Model>& meshes) {
+        static_assert(sizeof(Tri) == sizeof(rtq_triangle), "triangle records must be 64-B TriangleGPU");
+        static_assert(sizeof(Model) == sizeof(rtq_mesh_model), "model records must be 68-B MeshModelGPU");
+        ctx_->check(rtq_scene_update(
+            ctx_->get(), scene_, triangles.empty() ? nullptr : reinterpret_cast<const rtq_triangle*>(triangles.data()),
+            static_cast<uint32_t>(triangles.size()),
+            meshes.empty() ? nullptr : reinterpret_cast<const rtq_mesh_model*>(meshes.data()),
+            static_cast<uint32_t>(meshes.size())));
+        ctx_->check(rtq_bvh_refit(ctx_->get(), scene_));
+        fill(_InternalStruct._NbTriangles);
+    }
+
+    // getColor for device hit records into device RGBA32F pixels (rtq_shade).
+    void shade(const rtq_hit* hits_dev, uint64_t n, const rtq_material* materials_dev, uint32_t n_materials,
+               float* rgba_dev, uint32_t flags = 0, const float* overlay_dev = nullptr, void* stream = nullptr) const {
+        ctx_->check(rtq_shade(ctx_->get(), scene_, hits_dev, n, materials_dev, n_materials, overlay_dev, flags,
+                              rgba_dev, stream));
+    }
+
     uint32_t scene_id() const { return scene_; }
     const std::shared_ptr<Context>& context() const { return ctx_; }
 

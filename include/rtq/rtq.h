@@ -71,6 +71,15 @@ rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_t
                             uint32_t* scene_id);
 rtq_status rtq_scene_release(rtq_ctx* ctx, uint32_t scene_id);
 
+/* Per-frame scene edits (no counterpart in the reference, which builds its BVH
+ * once when it binds the scene, scene.cpp:110-187): new triangle vertices
+ * and/or model matrices, same counts as uploaded (NULL keeps an array), with
+ * the upload's checks.  Waits for work queued on the device.  A built scene
+ * becomes stale: trace, export and shading calls return RTQ_ENOTBUILT until
+ * rtq_bvh_refit or rtq_bvh_build. */
+rtq_status rtq_scene_update(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle* tris, uint32_t n_tris,
+                            const rtq_mesh_model* models, uint32_t n_models);
+
 /* ---- OBJ ingest (host only; no context or device needed) ------------------ */
 
 /* Mesh::load (mesh.cpp:186-263): the triangles of a Wavefront OBJ file in
@@ -113,6 +122,16 @@ void rtq_build_params_default(rtq_build_params* p);
  * PLOC (26-210) and the pre-order flatten (scene.cpp:189-208), then the
  * traversal layout.  Replaces BVH::BVH (bvh.cpp:11-24). */
 rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params* params);
+
+/* Refit: the built tree's topology kept (PLOC merges, leaf order, node
+ * indices), every box recomputed bottom-up for the scene's current vertices
+ * (leaf = triangle box, bvh.cpp:383-399; parent = merge of its children,
+ * bvh.cpp:401-413), then the reference node array, the traversal nodes and
+ * triangles and the compressed wide nodes rewritten in their buffers, so batch
+ * graphs captured before stay valid.  Learned order words are cleared.  Hit
+ * records stay exact (they never depend on the tree); traversal cost grows as
+ * the vertices move away from the ones the tree was built for. */
+rtq_status rtq_bvh_refit(rtq_ctx* ctx, uint32_t scene_id);
 
 typedef struct rtq_bvh_info {
     uint32_t n_tris;

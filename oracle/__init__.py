@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
             "orc_mlp_grad": (ctypes.c_double, [vp, vp, u64, vp]),
             "orc_order_features": (None, [vp, vp, u32, u32, vp]),
             "orc_ref_box_hit": (i32, [vp, vp, f32]),
+            "orc_refit_boxes": (None, [vp, vp, u32, vp, vp, vp, vp]),
             "orc_bvh_overlay": (None, [vp, u32, vp, u64, u32, i32, vp]),
             "orc_shade": (None, [vp, u32, vp, vp, u32, vp, u64, vp, u32, vp]),
         }
@@ -154,6 +155,21 @@ def build_ref_nodes(tris, models, pad_origin=False, radius=16):
     if passes < 0:
         raise RuntimeError("oracle PLOC failed")
     return out, passes
+
+
+def refit_boxes(tris, models, clusters) -> np.ndarray:
+    """Boxes of the cluster tree `clusters` (triangle_id, left, right) for the
+    vertices of `tris`: what rtq_bvh_refit computes."""
+    TRI, MOD, AABB, NODE, RAY, HIT = _dtypes()
+    tris = _c(tris, TRI)
+    models = _c(models, MOD)
+    n = len(tris)
+    out = np.zeros(2 * n - 1, AABB)
+    tid = _c(clusters["triangle_id"], np.uint32)
+    left = _c(clusters["left"], np.uint32)
+    right = _c(clusters["right"], np.uint32)
+    lib().orc_refit_boxes(_p(tris), _p(models), n, _p(tid), _p(left), _p(right), _p(out))
+    return out
 
 
 def tree_depth(nodes) -> int:

@@ -793,3 +793,16 @@ void orc_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n) {
     uint32_t acc = 0;
     for (uint64_t i = 0; i < n; ++i) { uint32_t x = in[i]; out[i] = acc; acc += x; }
 }
+
+/* rtq_bvh_refit: boxes of a kept cluster tree for new vertices.  Leaves
+ * (c < n) are triangle boxes (bvh.cpp:383-399); merged clusters (c >= n) the
+ * merge of their children (bvh.cpp:401-413), in increasing id order: a merge
+ * always gets a higher id than both children. */
+void orc_refit_boxes(const rtq_triangle* tris, const rtq_mesh_model* models, uint32_t n, const uint32_t* tri_id,
+                     const uint32_t* left, const uint32_t* right, rtq_aabb* boxes) {
+    for (uint32_t c = 0; c < n; ++c) {
+        const rtq_triangle* t = &tris[tri_id[c]];
+        boxes[c] = orc_triangle_box(t, &models[t->model_id]);
+    }
+    for (uint32_t c = n; c + 1 < 2 * n; ++c) boxes[c] = orc_merge(&boxes[left[c]], &boxes[right[c]]);
+}

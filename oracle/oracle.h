@@ -18,6 +18,9 @@
  *   orc_sort_pairs                   numpy lexsort
  *   orc_ploc_build                   brute-force NN on tiny N, mutual-pair
  *                                    invariants, 2N-1 clusters, box nesting
+ *   orc_refit_boxes                  equals orc_ploc_build's boxes on its own
+ *                                    tree; leaf = triangle box, parent = exact
+ *                                    merge (invariants)
  *   orc_flatten_preorder             pre-order invariants (root 0, L = i+1,
  *                                    every triangle in exactly one leaf)
  *   orc_refmt / orc_watertight       closed-form hits (axis-aligned triangles),
@@ -91,6 +94,10 @@ int orc_ploc_build(const rtq_triangle* tris, uint32_t n, const rtq_mesh_model* m
                    rtq_aabb* boxes, uint32_t* tri_id, uint32_t* left, uint32_t* right,
                    uint32_t* parent, uint32_t* leaf_count,
                    uint32_t* sorted_idx, uint32_t* sorted_codes);
+/* rtq_bvh_refit's boxes for a kept cluster tree (leaves from the triangles,
+ * merges in increasing cluster id); boxes holds 2n-1 entries. */
+void orc_refit_boxes(const rtq_triangle* tris, const rtq_mesh_model* models, uint32_t n, const uint32_t* tri_id,
+                     const uint32_t* left, const uint32_t* right, rtq_aabb* boxes);
 /* scene.cpp:189-208: pre-order from root 2n-2 (cluster 0 when n == 1). */
 int orc_flatten_preorder(uint32_t n, const rtq_aabb* boxes, const uint32_t* tri_id,
                          const uint32_t* left, const uint32_t* right, rtq_bvh_node* out);

@@ -62,6 +62,8 @@ void free_build(Scene* s) {
     release(s->tris);
     release(s->wide);
     s->built = false;
+    s->stale = false;
+    s->passes.clear();
     s->ordered = false;
     s->order_closest = false;
     s->order_any = false;
@@ -310,6 +312,31 @@ rtq_status rtq_synchronize(rtq_ctx* ctx) {
     return RTQ_OK;
 }
 
+// Upload checks (shared with rtq_scene_update): model ids in range, vertices
+// and matrix entries finite and bounded by 2^28 so every box surface area is
+// finite.
+static rtq_status check_scene_input(rtq_ctx* ctx, const char* what, const rtq_triangle* tris, uint32_t n_tris,
+                                    const rtq_mesh_model* models, uint32_t n_models) {
+    const float lim = 268435456.0f;
+    if (models)
+        for (uint32_t m = 0; m < n_models; ++m)
+            for (int k = 0; k < 16; ++k)
+                if (!finite_and_bounded(models[m].m[k], lim))
+                    return fail(ctx, RTQ_EINVAL, "%s: model %u has a non-finite or huge entry", what, m);
+    if (tris)
+        for (uint32_t i = 0; i < n_tris; ++i) {
+            const rtq_triangle& t = tris[i];
+            if (t.model_id >= n_models)
+                return fail(ctx, RTQ_EINVAL, "%s: triangle %u model id %u out of range", what, i, t.model_id);
+            const rtq_vec4* ps[3] = {&t.p0, &t.p1, &t.p2};
+            for (const rtq_vec4* p : ps)
+                if (!finite_and_bounded(p->x, lim) || !finite_and_bounded(p->y, lim) || !finite_and_bounded(p->z, lim) ||
+                    !finite_and_bounded(p->w, lim))
+                    return fail(ctx, RTQ_EINVAL, "%s: triangle %u has a non-finite or huge vertex", what, i);
+        }
+    return RTQ_OK;
+}
+
 rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_tris, const rtq_mesh_model* models,
                             uint32_t n_models, uint32_t* scene_id) {
     if (!ctx) return RTQ_EINVAL;
@@ -317,21 +344,8 @@ rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_t
     if (n_tris == 0) return fail(ctx, RTQ_EINVAL, "scene_upload: no triangles (the reference root id 2N-2 underflows)");
     if (n_models == 0) return fail(ctx, RTQ_EINVAL, "scene_upload: no mesh models");
     if (n_tris > kMaxTris) return fail(ctx, RTQ_ECAPACITY, "scene_upload: %u triangles exceeds %u", n_tris, kMaxTris);
-    // vertices and matrix entries bounded by 2^28 keep every box surface area finite
-    const float lim = 268435456.0f;
-    for (uint32_t m = 0; m < n_models; ++m)
-        for (int k = 0; k < 16; ++k)
-            if (!finite_and_bounded(models[m].m[k], lim))
-                return fail(ctx, RTQ_EINVAL, "scene_upload: model %u has a non-finite or huge entry", m);
-    for (uint32_t i = 0; i < n_tris; ++i) {
-        const rtq_triangle& t = tris[i];
-        if (t.model_id >= n_models) return fail(ctx, RTQ_EINVAL, "scene_upload: triangle %u model id %u out of range", i, t.model_id);
-        const rtq_vec4* ps[3] = {&t.p0, &t.p1, &t.p2};
-        for (const rtq_vec4* p : ps)
-            if (!finite_and_bounded(p->x, lim) || !finite_and_bounded(p->y, lim) || !finite_and_bounded(p->z, lim) ||
-                !finite_and_bounded(p->w, lim))
-                return fail(ctx, RTQ_EINVAL, "scene_upload: triangle %u has a non-finite or huge vertex", i);
-    }
+    rtq_status rs = check_scene_input(ctx, "scene_upload", tris, n_tris, models, n_models);
+    if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
     Scene* s = new Scene();
     s->n_tris = n_tris;
@@ -354,6 +368,58 @@ rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_t
     const uint32_t id = ctx->next_scene++;
     ctx->scenes[id] = s;
     *scene_id = id;
+    return RTQ_OK;
+}
+
+rtq_status rtq_scene_update(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle* tris, uint32_t n_tris,
+                            const rtq_mesh_model* models, uint32_t n_models) {
+    if (!ctx) return RTQ_EINVAL;
+    Scene* s = find_scene(ctx, scene_id);
+    if (!s) return fail(ctx, RTQ_EINVAL, "scene_update: unknown scene %u", scene_id);
+    if (tris && n_tris != s->n_tris)
+        return fail(ctx, RTQ_EINVAL, "scene_update: %u triangles, the scene has %u", n_tris, s->n_tris);
+    if (models && n_models != s->n_models)
+        return fail(ctx, RTQ_EINVAL, "scene_update: %u models, the scene has %u", n_models, s->n_models);
+    if (!tris && !models) return RTQ_OK;
+    // model ids of kept triangles are checked against the kept model count
+    rtq_status rs = check_scene_input(ctx, "scene_update", tris, s->n_tris, models, s->n_models);
+    if (rs != RTQ_OK) return rs;
+    RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    // work already queued on the scene may still read the arrays
+    RTQ_HIP(ctx, hipDeviceSynchronize());
+    if (tris)
+        RTQ_HIP(ctx, hipMemcpyAsync(s->tris_ref, tris, sizeof(rtq_triangle) * s->n_tris, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    if (models)
+        RTQ_HIP(ctx, hipMemcpyAsync(s->models, models, sizeof(DevModel) * s->n_models, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    hipLaunchKernelGGL(k_world, dim3(blocks_for(s->n_tris)), dim3(kThreads), 0, ctx->stream, s->tris_ref, s->models,
+                       s->n_tris, s->tri_world);
+    RTQ_HIP(ctx, hipGetLastError());
+    RTQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (s->built) {
+        s->built = false;
+        s->stale = true;
+    }
+    return RTQ_OK;
+}
+
+rtq_status rtq_bvh_refit(rtq_ctx* ctx, uint32_t scene_id) {
+    if (!ctx) return RTQ_EINVAL;
+    Scene* s = find_scene(ctx, scene_id);
+    if (!s) return fail(ctx, RTQ_EINVAL, "bvh_refit: unknown scene %u", scene_id);
+    if (!s->built && !s->stale) return fail(ctx, RTQ_ENOTBUILT, "bvh_refit: scene %u has no BVH", scene_id);
+    RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    RTQ_HIP(ctx, hipDeviceSynchronize());
+    s->built = false;
+    rtq_status rs = refit_scene(ctx, s);
+    if (rs != RTQ_OK) return rs;
+    // the traversal nodes were written with empty order words
+    s->ordered = false;
+    s->order_closest = false;
+    s->order_any = false;
+    s->stale = false;
+    s->built = true;
     return RTQ_OK;
 }
 

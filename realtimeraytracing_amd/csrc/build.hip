@@ -51,6 +51,42 @@ __device__ __forceinline__ void tri_box(const rtq_triangle& t, const DevModel& m
     }
 }
 
+// ------------------------------------------------------------- refit
+
+// Leaf cluster c < N keeps its triangle; its box is the triangle's box for the
+// current vertices (bvh.cpp:383-399), as k_leaf_init computes it.
+__global__ __launch_bounds__(kThreads) void k_refit_leaves(const rtq_triangle* __restrict__ tris,
+                                                           const DevModel* __restrict__ models,
+                                                           const uint32_t* __restrict__ ctri, uint32_t n,
+                                                           rtq_aabb* __restrict__ cbox) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const rtq_triangle t = tris[ctri[i]];
+    float lo[3], hi[3];
+    tri_box(t, models[t.model_id], lo, hi);
+    rtq_aabb a{};
+    for (int k = 0; k < 3; ++k) { a.min[k] = lo[k]; a.max[k] = hi[k]; }
+    cbox[i] = a;
+}
+
+// Merged clusters [first, first + count) of one PLOC pass: the box is the
+// merge of the children's (left operand first, as k_ploc_merge).
+__global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32_t count,
+                                                          const uint32_t* __restrict__ cleft,
+                                                          const uint32_t* __restrict__ cright,
+                                                          rtq_aabb* __restrict__ cbox) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t id = first + i;
+    const rtq_aabb l = cbox[cleft[id]], r = cbox[cright[id]];
+    rtq_aabb a{};
+    for (int k = 0; k < 3; ++k) {
+        a.min[k] = smin(l.min[k], r.min[k]);
+        a.max[k] = smax(l.max[k], r.max[k]);
+    }
+    cbox[id] = a;
+}
+
 // ------------------------------------------------------------- scene bounds
 
 __global__ __launch_bounds__(kThreads) void k_bounds_partial(const rtq_triangle* __restrict__ tris,
@@ -481,6 +517,82 @@ struct BuildTemps {
     }
 };
 
+// Steps after PLOC, shared by the build and the refit: pre-order placement,
+// the reference node array and leaf-ordered triangles, the box pad from the
+// scene extent, the binary traversal nodes and (node_width 8) the compressed
+// wide nodes.  Buffers that exist already are written in place (a refit keeps
+// their addresses, so captured batch graphs stay valid).
+rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipStream_t st) {
+    const uint32_t n = s->n_tris;
+    const uint32_t nc = s->n_clusters;
+    BuildTemps tmp;
+    uint32_t *pos = nullptr, *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr,
+             *bidx = nullptr, *ntrav = nullptr;
+    RTQ_HIP(ctx, tmp.get(&pos, nc));
+    RTQ_HIP(ctx, tmp.get(&fl, nc));
+    RTQ_HIP(ctx, tmp.get(&dep, nc));
+    RTQ_HIP(ctx, tmp.get(&maxd, 2));
+    RTQ_HIP(ctx, tmp.get(&flag, nc));
+    RTQ_HIP(ctx, tmp.get(&bidx, nc));
+    RTQ_HIP(ctx, tmp.get(&ntrav, 1));
+    RTQ_HIP(ctx, hipMemsetAsync(maxd, 0, sizeof(uint32_t) * 2, st));
+    const uint32_t root = nc - 1;
+    hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
+    RTQ_HIP(ctx, hipGetLastError());
+    for (auto it = s->passes.rbegin(); it != s->passes.rend(); ++it) {
+        hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(it->second)), dim3(kThreads), 0, st, it->first,
+                           it->second, s->cleft, s->cright, s->ccount, pos, fl, dep);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_write_ref, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->cbox, s->ctri,
+                       s->cleft, s->cright, pos, fl, dep, s->tri_world, s->ref_nodes, s->tris, maxd);
+    RTQ_HIP(ctx, hipGetLastError());
+
+    rtq_aabb h_bounds[2];
+    uint32_t h_depth = 0;
+    RTQ_HIP(ctx, hipMemcpyAsync(h_bounds, d_bounds, sizeof h_bounds, hipMemcpyDeviceToHost, st));
+    RTQ_HIP(ctx, hipMemcpyAsync(&h_depth, maxd, sizeof h_depth, hipMemcpyDeviceToHost, st));
+    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    float extent = 0.0f;
+    for (int k = 0; k < 3; ++k)
+        extent = std::max(extent, std::max(std::fabs(h_bounds[0].min[k]), std::fabs(h_bounds[0].max[k])));
+    s->box_pad = std::max(std::ldexp(extent, -18), 1e-30f);
+    s->depth = h_depth;
+
+    const uint32_t max_leaf = s->params.max_leaf_size;
+    if (n == 1) {
+        s->n_trav = 1;
+        if (!s->nodes) RTQ_HIP(ctx, dev_alloc(&s->nodes, 4));
+        if (!s->node_aux) RTQ_HIP(ctx, dev_alloc(&s->node_aux, 1));
+        hipLaunchKernelGGL(k_write_single, dim3(1), dim3(1), 0, st, s->cbox, s->box_pad, s->nodes, s->node_aux);
+        RTQ_HIP(ctx, hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
+                           s->cleft, s->ccount, pos, flag);
+        RTQ_HIP(ctx, hipGetLastError());
+        rtq_status rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
+        if (rs != RTQ_OK) return rs;
+        uint32_t h_ntrav = 0;
+        RTQ_HIP(ctx, hipMemcpyAsync(&h_ntrav, ntrav, sizeof h_ntrav, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        if (s->nodes && h_ntrav != s->n_trav)
+            return fail(ctx, RTQ_EINTERNAL, "refit: %u traversal nodes, the tree has %u", h_ntrav, s->n_trav);
+        s->n_trav = h_ntrav;
+        if (!s->nodes) RTQ_HIP(ctx, dev_alloc(&s->nodes, 4ull * h_ntrav));
+        if (!s->node_aux) RTQ_HIP(ctx, dev_alloc(&s->node_aux, h_ntrav));
+        hipLaunchKernelGGL(k_write_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
+                           s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, dep, s->nodes,
+                           s->node_aux);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    if (s->params.node_width == 8) {
+        rtq_status rs = build_wide(ctx, s, fl, st);
+        if (rs != RTQ_OK) return rs;
+    }
+    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    return RTQ_OK;
+}
+
 }  // namespace
 
 rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
@@ -573,70 +685,11 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     }
     if (n_total != nc) return fail(ctx, RTQ_EINTERNAL, "PLOC produced %u clusters, expected %u", n_total, nc);
     s->ploc_passes = (uint32_t)pass_ranges.size();
+    s->passes = pass_ranges;
 
-    // 3. pre-order placement, reference node array, leaf-ordered triangles
-    uint32_t *pos = nullptr, *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr,
-             *bidx = nullptr, *ntrav = nullptr;
-    RTQ_HIP(ctx, tmp.get(&pos, nc));
-    RTQ_HIP(ctx, tmp.get(&fl, nc));
-    RTQ_HIP(ctx, tmp.get(&dep, nc));
-    RTQ_HIP(ctx, tmp.get(&maxd, 2));
-    RTQ_HIP(ctx, tmp.get(&flag, nc));
-    RTQ_HIP(ctx, tmp.get(&bidx, nc));
-    RTQ_HIP(ctx, tmp.get(&ntrav, 1));
-    RTQ_HIP(ctx, hipMemsetAsync(maxd, 0, sizeof(uint32_t) * 2, st));
-    const uint32_t root = nc - 1;
-    hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
-    RTQ_HIP(ctx, hipGetLastError());
-    for (auto it = pass_ranges.rbegin(); it != pass_ranges.rend(); ++it) {
-        hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(it->second)), dim3(kThreads), 0, st, it->first,
-                           it->second, s->cleft, s->cright, s->ccount, pos, fl, dep);
-        RTQ_HIP(ctx, hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_write_ref, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->cbox, s->ctri,
-                       s->cleft, s->cright, pos, fl, dep, s->tri_world, s->ref_nodes, s->tris, maxd);
-    RTQ_HIP(ctx, hipGetLastError());
-
-    // 4. traversal layout: padding from the scene extent
-    rtq_aabb h_bounds[2];
-    uint32_t h_depth = 0;
-    RTQ_HIP(ctx, hipMemcpyAsync(h_bounds, d_bounds, sizeof h_bounds, hipMemcpyDeviceToHost, st));
-    RTQ_HIP(ctx, hipMemcpyAsync(&h_depth, maxd, sizeof h_depth, hipMemcpyDeviceToHost, st));
-    RTQ_HIP(ctx, hipStreamSynchronize(st));
-    float extent = 0.0f;
-    for (int k = 0; k < 3; ++k)
-        extent = std::max(extent, std::max(std::fabs(h_bounds[0].min[k]), std::fabs(h_bounds[0].max[k])));
-    s->box_pad = std::max(std::ldexp(extent, -18), 1e-30f);
-    s->depth = h_depth;
-
-    const uint32_t max_leaf = prm.max_leaf_size;
-    if (n == 1) {
-        s->n_trav = 1;
-        RTQ_HIP(ctx, dev_alloc(&s->nodes, 4));
-        RTQ_HIP(ctx, dev_alloc(&s->node_aux, 1));
-        hipLaunchKernelGGL(k_write_single, dim3(1), dim3(1), 0, st, s->cbox, s->box_pad, s->nodes, s->node_aux);
-        RTQ_HIP(ctx, hipGetLastError());
-    } else {
-        hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
-                           s->cleft, s->ccount, pos, flag);
-        RTQ_HIP(ctx, hipGetLastError());
-        rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
-        if (rs != RTQ_OK) return rs;
-        uint32_t h_ntrav = 0;
-        RTQ_HIP(ctx, hipMemcpyAsync(&h_ntrav, ntrav, sizeof h_ntrav, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipStreamSynchronize(st));
-        s->n_trav = h_ntrav;
-        RTQ_HIP(ctx, dev_alloc(&s->nodes, 4ull * h_ntrav));
-        RTQ_HIP(ctx, dev_alloc(&s->node_aux, h_ntrav));
-        hipLaunchKernelGGL(k_write_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
-                           s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, dep, s->nodes,
-                           s->node_aux);
-        RTQ_HIP(ctx, hipGetLastError());
-    }
-    if (prm.node_width == 8) {
-        rs = build_wide(ctx, s, fl, st);
-        if (rs != RTQ_OK) return rs;
-    }
+    // 3-4. node arrays of every layout
+    rs = write_layouts(ctx, s, d_bounds, st);
+    if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipEventRecord(ev1, st));
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     float ms = 0.0f;
@@ -644,6 +697,29 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     s->build_ms = ms;
     s->built = true;
     return RTQ_OK;
+}
+
+// New boxes for the kept cluster tree (rtq_bvh_refit): leaf boxes from the
+// current vertices, then the merges of each PLOC pass in build order (a
+// merge's children come from earlier passes), then every node array again,
+// written into the buffers it already has.
+rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
+    hipStream_t st = ctx->stream;
+    const uint32_t n = s->n_tris;
+    BuildTemps tmp;
+    rtq_aabb* d_bounds = nullptr;
+    RTQ_HIP(ctx, tmp.get(&d_bounds, 2));
+    hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, s->ctri,
+                       n, s->cbox);
+    RTQ_HIP(ctx, hipGetLastError());
+    for (const auto& pr : s->passes) {
+        hipLaunchKernelGGL(k_refit_merge, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
+                           s->cleft, s->cright, s->cbox);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    rtq_status rs = compute_bounds(ctx, s, s->params.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
+    if (rs != RTQ_OK) return rs;
+    return write_layouts(ctx, s, d_bounds, st);
 }
 
 rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_codes) {

@@ -65,6 +65,10 @@ struct Scene {
     uint32_t wide_depth = 0;            // levels of the wide tree
     uint32_t depth = 0;
     uint32_t ploc_passes = 0;
+    // (first cluster id, count) of the merges of each PLOC pass, in build order
+    std::vector<std::pair<uint32_t, uint32_t>> passes;
+    // vertices changed since the last build or refit (rtq_scene_update)
+    bool stale = false;
     float box_pad = 0.0f;
     float build_ms = 0.0f;
 };
@@ -143,6 +147,8 @@ rtq_status scratch_alloc(rtq_ctx* ctx, TraceScratch* sc);
 void scratch_free(TraceScratch* sc);
 // Compressed 8-wide nodes from the cluster arrays (fl: first leaf rank per cluster).
 rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st);
+// New boxes for the kept topology after rtq_scene_update (build.hip).
+rtq_status refit_scene(rtq_ctx* ctx, Scene* s);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
                             uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st);

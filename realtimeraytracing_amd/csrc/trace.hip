@@ -665,9 +665,12 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
 }
 
 // Stack entries a traversal can hold: binary nodes push one entry per tree
-// level below the root, a wide node at most 7 per level.
+// level below the root, a wide node at most 7 per level.  The wide bound uses
+// the binary depth (never less than the wide depth): a refit keeps the binary
+// topology but may collapse the wide tree differently, and graphs captured
+// before it keep their spill buffers.
 uint64_t stack_bound(const Scene* s, bool wide) {
-    return wide ? 7ull * s->wide_depth + 2 : (uint64_t)s->depth + 2;
+    return wide ? 7ull * s->depth + 2 : (uint64_t)s->depth + 2;
 }
 
 template <int LEAF, int MODE, bool STATS>

@@ -190,7 +190,8 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st
     RTQ_HIP(ctx, get(&nexp, cap));
     RTQ_HIP(ctx, get(&offs, cap));
     RTQ_HIP(ctx, get(&total, 1));
-    RTQ_HIP(ctx, dev_alloc(&s->wide, 8ull * cap));
+    // a refit writes the same buffer again (its size depends on the topology only)
+    if (!s->wide) RTQ_HIP(ctx, dev_alloc(&s->wide, 8ull * cap));
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, items, root);
     RTQ_HIP(ctx, hipGetLastError());
     uint32_t level_items = 1, level_base = 0, levels = 0;

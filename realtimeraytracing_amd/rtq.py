@@ -56,6 +56,7 @@ EXPORTED = [
     "rtq_order_model_create", "rtq_order_model_destroy", "rtq_order_model_log", "rtq_order_model_clear_samples",
     "rtq_order_model_train", "rtq_order_model_apply", "rtq_order_model_save", "rtq_order_model_load",
     "rtq_order_model_info", "rtq_shade", "rtq_bvh_overlay", "rtq_obj_load", "rtq_obj_parse",
+    "rtq_scene_update", "rtq_bvh_refit",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -129,6 +130,8 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_synchronize": (st, [vp]),
         "rtq_scene_upload": (st, [vp, vp, u32, vp, u32, ctypes.POINTER(u32)]),
         "rtq_scene_release": (st, [vp, u32]),
+        "rtq_scene_update": (st, [vp, u32, vp, u32, vp, u32]),
+        "rtq_bvh_refit": (st, [vp, u32]),
         "rtq_build_params_default": (None, [ctypes.POINTER(BuildParams)]),
         "rtq_bvh_build": (st, [vp, u32, ctypes.POINTER(BuildParams)]),
         "rtq_bvh_info_get": (st, [vp, u32, ctypes.POINTER(BvhInfo)]),
@@ -400,6 +403,17 @@ class Scene:
         p = BuildParams(search_radius, max_leaf_size, flags, node_width)
         self.ctx.check(self.ctx.lib.rtq_bvh_build(self.ctx.h, self.id, ctypes.byref(p)))
         return self.info()
+
+    def update(self, tris: np.ndarray | None = None, models: np.ndarray | None = None) -> None:
+        """New vertices and/or model matrices (same counts); the BVH is stale until refit or build."""
+        t = None if tris is None else np.ascontiguousarray(tris, dtype=TRIANGLE_DTYPE)
+        m = None if models is None else np.ascontiguousarray(models, dtype=MODEL_DTYPE)
+        self.ctx.check(self.ctx.lib.rtq_scene_update(self.ctx.h, self.id, _ptr(t), 0 if t is None else len(t),
+                                                     _ptr(m), 0 if m is None else len(m)))
+
+    def refit(self) -> None:
+        """New boxes for the kept tree (rtq_bvh_refit)."""
+        self.ctx.check(self.ctx.lib.rtq_bvh_refit(self.ctx.h, self.id))
 
     def info(self) -> BvhInfo:
         out = BvhInfo()

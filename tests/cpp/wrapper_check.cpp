@@ -91,6 +91,37 @@ int main() {
     }
     CHECK(threw);
 
+    // ---- scene edit: the grid lifted to y = 0.5 by its model matrix, refit --
+    {
+        std::vector<MeshModelGPU> lifted = models;
+        lifted[0].m[13] = 0.5f;                      // column 3, row 1: translation y
+        bvh.update(std::vector<TriangleGPU>{}, lifted);
+        CHECK(p._Clusters[2 * n - 2]->box.min[1] == 0.5f && p._Clusters[2 * n - 2]->box.max[1] == 0.5f);
+        for (const rtq_hit& h : bvh.trace(rays)) CHECK(h.id >= 0 && h.t == 0.5f);
+        // shading of those hits with one material (getColor: base + colour)
+        rtq_hit* d_h = nullptr;
+        rtq_material* d_m = nullptr;
+        float* d_px = nullptr;
+        std::vector<rtq_hit> lifted_hits = bvh.trace(rays);
+        const rtq_material mat{{0.2f, 0.3f, 0.1f, 1.0f}};
+        CHECK(hipMalloc(&d_h, lifted_hits.size() * sizeof(rtq_hit)) == hipSuccess);
+        CHECK(hipMalloc(&d_m, sizeof mat) == hipSuccess);
+        CHECK(hipMalloc(&d_px, lifted_hits.size() * 4 * sizeof(float)) == hipSuccess);
+        CHECK(hipMemcpy(d_h, lifted_hits.data(), lifted_hits.size() * sizeof(rtq_hit), hipMemcpyHostToDevice) ==
+              hipSuccess);
+        CHECK(hipMemcpy(d_m, &mat, sizeof mat, hipMemcpyHostToDevice) == hipSuccess);
+        bvh.shade(d_h, lifted_hits.size(), d_m, 1, d_px);
+        CHECK(rtq_synchronize(ctx->get()) == RTQ_OK);
+        std::vector<float> px(lifted_hits.size() * 4);
+        CHECK(hipMemcpy(px.data(), d_px, px.size() * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess);
+        for (size_t i = 0; i < lifted_hits.size(); ++i)
+            CHECK(px[4 * i] == 0.2f && px[4 * i + 1] == 0.3f && px[4 * i + 2] == 0.1f && px[4 * i + 3] == 2.0f);
+        (void)hipFree(d_h);
+        (void)hipFree(d_m);
+        (void)hipFree(d_px);
+        bvh.update(std::vector<TriangleGPU>{}, models);   // back to y = 0 for the checks below
+    }
+
     // ---- frame batch + autotuner + learned order on device buffers --------
     // a second scene: the grid plus a cloud of small occluders above it
     std::vector<TriangleGPU> occ_tris = tris;

@@ -1,0 +1,171 @@
+"""Scene edits and refit (rtq_scene_update / rtq_bvh_refit): the oracle's refit
+boxes are pinned against its own build and by the box invariants; on the GPU
+a refit of unchanged vertices reproduces the build bit for bit, a refit of
+moved vertices gives the oracle's boxes and node array, hits equal brute force,
+and batch graphs captured before a refit stay valid."""
+import numpy as np
+import pytest
+
+import oracle
+from realtimeraytracing_amd import (HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_ANY, RTQ_TRACE_BINARY, RtqError, devmem,
+                                    scenes)
+from realtimeraytracing_amd.rtq import RTQ_EINVAL, RTQ_ENOTBUILT
+from conftest import first_diff, hits_equal
+
+
+def moved(tris, seed=5, amount=0.02):
+    """Every vertex moved by a seeded offset (same triangles, new positions)."""
+    out = tris.copy()
+    i = np.arange(len(tris), dtype=np.uint64)
+    for k, name in enumerate(("p0", "p1", "p2")):
+        off = np.stack([scenes.u01(seed, 20 + 3 * k + c, i) for c in range(3)], axis=1) * 2.0 - 1.0
+        out[name][:, :3] = (out[name][:, :3].astype(np.float64) + amount * off).astype(np.float32)
+    return out
+
+
+def box_bits(b):
+    return np.ascontiguousarray(b[["min", "max"]]).view(np.uint8)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 300, 5000])
+def test_oracle_refit_matches_build_and_invariants(n):
+    tris = scenes.soup(n, seed=n)
+    models = scenes.identity_models()
+    cl = oracle.ploc_build(tris, models)
+    # the same vertices give the build's boxes
+    assert np.array_equal(box_bits(oracle.refit_boxes(tris, models, cl)), box_bits(cl["boxes"]))
+    # moved vertices: leaf = triangle box, parent = merge of its children
+    new = moved(tris)
+    b = oracle.refit_boxes(new, models, cl)
+    v = oracle.world_vertices(new, models).reshape(-1, 3, 3)
+    leaf = v[cl["triangle_id"][:n]]
+    assert np.array_equal(b["min"][:n], leaf.min(axis=1)) and np.array_equal(b["max"][:n], leaf.max(axis=1))
+    for c in range(n, 2 * n - 1):
+        l, r = cl["left"][c], cl["right"][c]
+        assert np.array_equal(b["min"][c], np.minimum(b["min"][l], b["min"][r]))
+        assert np.array_equal(b["max"][c], np.maximum(b["max"][l], b["max"][r]))
+
+
+def rays_for(n=4096, seed=3):
+    return scenes.random_rays(n, seed, (-0.1, -0.1, -0.1), (1.1, 1.1, 1.1))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,width", [(1, 2), (5000, 2), (20000, 8)])
+def test_refit_unchanged_is_the_build(rtq_ctx, n, width):
+    tris = scenes.soup(n, seed=n)
+    sc = rtq_ctx.upload(tris, scenes.identity_models())
+    try:
+        info0 = sc.build(16, 4, 0, node_width=width)
+        ref0, (trav0, rng0), cl0 = sc.export_ref(), sc.export_trav(), sc.export_clusters()
+        rays = rays_for()
+        h0 = sc.trace_host(rays)
+        sc.update(tris)
+        sc.refit()
+        info1 = sc.info()
+        assert hits_equal(sc.export_ref(), ref0)
+        trav1, rng1 = sc.export_trav()
+        assert np.array_equal(trav1.view(np.uint32), trav0.view(np.uint32)) and np.array_equal(rng1, rng0)
+        assert np.array_equal(box_bits(sc.export_clusters()["boxes"]), box_bits(cl0["boxes"]))
+        assert (info1.n_wide_nodes, info1.wide_depth, info1.depth, info1.box_pad) == (
+            info0.n_wide_nodes, info0.wide_depth, info0.depth, info0.box_pad)
+        assert hits_equal(sc.trace_host(rays), h0)
+    finally:
+        sc.release()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("width", [2, 8])
+def test_refit_moved_vertices(rtq_ctx, width):
+    n = 20000
+    tris = scenes.soup(n, seed=9)
+    models = scenes.identity_models()
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        sc.build(16, 4, 0, node_width=width)
+        cl = sc.export_clusters()
+        new = moved(tris, amount=0.05)
+        sc.update(new)
+        with pytest.raises(RtqError) as e:
+            sc.trace_host(rays_for(16))
+        assert e.value.status == RTQ_ENOTBUILT
+        sc.refit()
+        # boxes of the kept tree and the node array, as the oracle computes them
+        boxes = oracle.refit_boxes(new, models, cl)
+        got = sc.export_clusters()
+        assert np.array_equal(box_bits(got["boxes"]), box_bits(boxes))
+        assert all(np.array_equal(got[k], cl[k]) for k in ("triangle_id", "left", "right", "parent"))
+        ref = oracle.flatten_preorder(n, dict(cl, boxes=boxes))
+        assert hits_equal(sc.export_ref(), ref), first_diff(sc.export_ref(), ref)
+        # hits equal brute force over the new vertices, closest and any hit
+        rays = rays_for()
+        v9 = oracle.world_vertices(new, models)
+        for flags, mode in ((0, oracle.MODE_CLOSEST), (RTQ_TRACE_BINARY, oracle.MODE_CLOSEST)):
+            g = sc.trace_host(rays, flags)
+            b = oracle.trace_brute(v9, rays, mode=mode)
+            assert hits_equal(g, b), first_diff(g, b)
+        g = sc.trace_host(rays, RTQ_TRACE_ANY)
+        b = oracle.trace_brute(v9, rays, mode=oracle.MODE_ANY)
+        assert np.array_equal(g["id"] >= 0, b["id"] >= 0)
+    finally:
+        sc.release()
+
+
+@pytest.mark.gpu
+def test_model_update_and_graph_replay(rtq_ctx, torch_mod):
+    """Moving a mesh by its model matrix, refitting, and replaying a batch graph
+    captured before the refit: the replay sees the new boxes."""
+    tris = scenes.soup(8000, seed=4)
+    models = scenes.identity_models()
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        sc.build(16, 4, 0, node_width=8)
+        rays = rays_for(8192, seed=8)
+        n = len(rays)
+        rd = devmem.to_device(rays)
+        hd = devmem.empty(n, HIT_DTYPE)
+        torch_mod.cuda.synchronize()
+        items = rtq_ctx.items([(sc, rd.data_ptr(), n, hd.data_ptr(), 0)])
+        graph = rtq_ctx.batch(items)
+        try:
+            for step in range(3):
+                m = scenes.identity_models()
+                m["m"][0][12:15] = (0.1 * step, -0.05 * step, 0.02 * step)     # translation column
+                sc.update(models=m)
+                sc.refit()
+                graph.launch()
+                rtq_ctx.synchronize()
+                got = devmem.to_host(hd, HIT_DTYPE)
+                v9 = oracle.world_vertices(tris, m)
+                ref = oracle.trace_brute(v9, rays)
+                assert hits_equal(got, ref), (step, first_diff(got, ref))
+                assert 0.05 < (got["id"] >= 0).mean() < 0.95
+        finally:
+            graph.destroy()
+    finally:
+        sc.release()
+
+
+@pytest.mark.gpu
+def test_update_errors(rtq_ctx):
+    tris = scenes.soup(100, seed=1)
+    sc = rtq_ctx.upload(tris, scenes.identity_models())
+    try:
+        with pytest.raises(RtqError) as e:
+            sc.refit()                                   # nothing built yet
+        assert e.value.status == RTQ_ENOTBUILT
+        sc.build(16, 4, 0)
+        with pytest.raises(RtqError) as e:
+            sc.update(tris[:50])
+        assert e.value.status == RTQ_EINVAL
+        with pytest.raises(RtqError) as e:
+            sc.update(models=scenes.identity_models(2))
+        assert e.value.status == RTQ_EINVAL
+        bad = tris.copy()
+        bad["p1"][3, 0] = np.inf
+        with pytest.raises(RtqError) as e:
+            sc.update(bad)
+        assert e.value.status == RTQ_EINVAL
+        sc.trace_host(rays_for(16))                      # rejected updates leave the scene as it was
+    finally:
+        sc.release()

@@ -47,10 +47,12 @@ constexpr uint32_t kCounterWords = kArrive + kPartStride;
 #define RTQ_STACK_LDS 16
 #endif
 constexpr uint32_t kStackLds = RTQ_STACK_LDS;
-// Optional VGPR budget of the trace kernels; occupancy is 512 / budget waves
-// per SIMD.
+// Optional VGPR budget of the trace kernels (occupancy is 512 / budget waves
+// per SIMD), or an occupancy the compiler must fit (waves per SIMD).
 #ifdef RTQ_TRACE_VGPRS
 #define RTQ_TRACE_OCCUPANCY __attribute__((amdgpu_num_vgpr(RTQ_TRACE_VGPRS)))
+#elif defined(RTQ_TRACE_WAVES)
+#define RTQ_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RTQ_TRACE_WAVES)))
 #else
 #define RTQ_TRACE_OCCUPANCY
 #endif
@@ -218,8 +220,10 @@ enum Layout : int {
 
 // MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: rays come from
 // several items (scenes), each lane takes its scene and buffers from its item.
+// The body of the trace kernels below (k_trace, and k_trace_occ with an
+// occupancy hint).
 template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
-__global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(TraceArgs a, ItemTable<MULTI> table) {
+__device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<MULTI>& table) {
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
     const uint32_t tid = threadIdx.x;
@@ -620,14 +624,52 @@ __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(Tra
     }
 }
 
+template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
+__global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(TraceArgs a, ItemTable<MULTI> table) {
+    trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI>(a, table);
+}
+
+// The same kernel with an occupancy (waves per SIMD) the register allocation
+// must fit.
+template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI, int WAVES>
+__global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace_occ(
+    TraceArgs a, ItemTable<MULTI> table) {
+    trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI>(a, table);
+}
+
+// Occupancy hints by layout (0 = the compiler's own choice), from timings on
+// the 2M-triangle terrain: the batched binary kernel at 7 waves per SIMD
+// instead of 6 (74 -> 72 VGPRs, no spill), the single-scene 8-wide kernels at
+// RTQ_OCC_WIDE instead of 5-6; the single-scene binary kernels were slower
+// when pushed from 7 to 8 waves.
+#ifndef RTQ_OCC_BATCH
+#define RTQ_OCC_BATCH 7
+#endif
+#ifndef RTQ_OCC_WIDE
+#define RTQ_OCC_WIDE 6
+#endif
+template <int LAYOUT, bool MULTI>
+constexpr int occupancy_hint() {
+    return MULTI ? (LAYOUT == kBinary ? RTQ_OCC_BATCH : 0) : (LAYOUT == kWide ? RTQ_OCC_WIDE : 0);
+}
+
+template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
+auto trace_kernel() {
+    constexpr int w = occupancy_hint<LAYOUT, MULTI>();
+    if constexpr (w > 0)
+        return &k_trace_occ<LEAF, MODE, STATS, LAYOUT, MULTI, w>;
+    else
+        return &k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>;
+}
+
 // Persistent grid sized from occupancy, spill column sized from the stack
 // bound `depth_entries`, one launch (or only the scratch sizing).
 template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
 rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const TraceArgs& base,
                   const ItemTable<MULTI>& table, uint32_t n, bool size_only, hipStream_t st) {
+    const auto kern = trace_kernel<LEAF, MODE, STATS, LAYOUT, MULTI>();
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>,
-                                                               kTraceThreads, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTraceThreads, 0);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
@@ -658,8 +700,7 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
     args.chunk = ctx->chunk;
     args.slack = ctx->spec_slack;
     args.leaf_slack = ctx->leaf_slack;
-    hipLaunchKernelGGL((k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>), dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st,
-                       args, table);
+    hipLaunchKernelGGL(kern, dim3((uint32_t)blocks), dim3(kTraceThreads), 0, st, args, table);
     RTQ_HIP(ctx, hipGetLastError());
     return RTQ_OK;
 }

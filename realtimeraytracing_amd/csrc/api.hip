@@ -59,6 +59,8 @@ void free_build(Scene* s) {
     release(s->ref_nodes);
     release(s->nodes);
     release(s->node_aux);
+    release(s->top);
+    s->n_top = 0;
     release(s->tris);
     release(s->wide);
     s->built = false;

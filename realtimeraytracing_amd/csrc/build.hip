@@ -87,6 +87,46 @@ __global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32
     cbox[id] = a;
 }
 
+// ------------------------------------------------------------- top of the tree
+
+// One thread: BFS from the root over internal-node refs, at most kTopNodes
+// nodes; their 64-B records are copied with refs into the set rewritten to
+// kTopBit | slot (leaf refs and refs leaving the set unchanged).
+__global__ void k_top_nodes(const float4* __restrict__ nodes, float4* __restrict__ top,
+                            uint32_t* __restrict__ n_top) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t q[kTopNodes > 0 ? kTopNodes : 1];
+    uint32_t n = 0;
+    q[n++] = 0;
+    for (uint32_t i = 0; i < n && n < kTopNodes; ++i) {
+        const float4 w = nodes[4ull * q[i] + 3];
+        const uint32_t r0 = __float_as_uint(w.x), r1 = __float_as_uint(w.y);
+        if (!(r0 & kLeafBit) && n < kTopNodes) q[n++] = r0;
+        if (!(r1 & kLeafBit) && n < kTopNodes) q[n++] = r1;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const float4* src = nodes + 4ull * q[i];
+        float4* dst = top + 4ull * i;
+        dst[0] = src[0];
+        dst[1] = src[1];
+        dst[2] = src[2];
+        float4 w = src[3];
+        uint32_t r[2] = {__float_as_uint(w.x), __float_as_uint(w.y)};
+        for (int c = 0; c < 2; ++c) {
+            if (r[c] & kLeafBit) continue;
+            for (uint32_t j = 1; j < n; ++j)
+                if (q[j] == r[c]) {
+                    r[c] = kTopBit | j;
+                    break;
+                }
+        }
+        w.x = __uint_as_float(r[0]);
+        w.y = __uint_as_float(r[1]);
+        dst[3] = w;
+    }
+    *n_top = n;
+}
+
 // ------------------------------------------------------------- scene bounds
 
 __global__ __launch_bounds__(kThreads) void k_bounds_partial(const rtq_triangle* __restrict__ tris,
@@ -589,6 +629,8 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
         rtq_status rs = build_wide(ctx, s, fl, st);
         if (rs != RTQ_OK) return rs;
     }
+    rtq_status rs = write_top(ctx, s, st);
+    if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     return RTQ_OK;
 }
@@ -696,6 +738,24 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     RTQ_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
     s->build_ms = ms;
     s->built = true;
+    return RTQ_OK;
+}
+
+rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st) {
+    if (kTopNodes == 0) return RTQ_OK;
+    // the set depends on the topology only, so a refit or an order update
+    // rewrites the same buffer with the same node count
+    if (!s->top) RTQ_HIP(ctx, dev_alloc(&s->top, 4ull * kTopNodes));
+    uint32_t* d_n = nullptr;
+    RTQ_HIP(ctx, dev_alloc(&d_n, 1));
+    hipLaunchKernelGGL(k_top_nodes, dim3(1), dim3(1), 0, st, s->nodes, s->top, d_n);
+    hipError_t e = hipGetLastError();
+    uint32_t n = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, d_n, sizeof n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_n);
+    if (e != hipSuccess) return hip_fail(ctx, e, "write_top");
+    s->n_top = n;
     return RTQ_OK;
 }
 

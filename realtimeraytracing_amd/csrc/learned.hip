@@ -771,6 +771,9 @@ rtq_status rtq_order_apply(rtq_ctx* ctx, uint32_t scene_id, const float* params_
                            s->n_trav, zmin(confidence_closest), zmin(confidence_any));
     }
     RTQ_HIP(ctx, hipGetLastError());
+    // the LDS copy of the top of the tree carries the order words too
+    rtq_status rs = write_top(ctx, s, st);
+    if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     s->ordered = params_dev != nullptr;
     s->order_closest = s->ordered && confidence_closest < 1.0f;

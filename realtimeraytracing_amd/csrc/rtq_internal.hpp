@@ -22,6 +22,16 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kMaxTris = (1u << 28) - 16u;
 constexpr uint32_t kMaxLeaf = 8u;
 
+// Top of the binary traversal tree, staged in LDS by every trace workgroup:
+// the first kTopNodes internal nodes in BFS order from the root, copied with
+// the child refs that stay inside this set rewritten to kTopBit | slot
+// (internal node indices stay below 2^28, so bit 30 is free in node refs).
+#ifndef RTQ_TOP_NODES
+#define RTQ_TOP_NODES 63
+#endif
+constexpr uint32_t kTopNodes = RTQ_TOP_NODES;
+constexpr uint32_t kTopBit = 0x40000000u;
+
 __host__ __device__ inline uint32_t leaf_ref(uint32_t first, uint32_t count) {
     return kLeafBit | (first << 3) | (count - 1u);
 }
@@ -58,6 +68,8 @@ struct Scene {
     bool ordered = false;               // learned child-order words stored in the nodes
     bool order_closest = false;         // ... with decisions for closest-hit rays
     bool order_any = false;             // ... with decisions for any-hit rays
+    float4* top = nullptr;              // top-of-tree copy, kTopNodes x 64 B (refs rewritten)
+    uint32_t n_top = 0;                 // nodes in it (fewer for small trees)
     float4* tris = nullptr;             // traversal triangles, leaf order, 3 float4 (48 B) each
     uint32_t n_trav = 0;
     float4* wide = nullptr;             // compressed 8-wide nodes, 8 float4 (128 B) each
@@ -149,6 +161,9 @@ void scratch_free(TraceScratch* sc);
 rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st);
 // New boxes for the kept topology after rtq_scene_update (build.hip).
 rtq_status refit_scene(rtq_ctx* ctx, Scene* s);
+// (Re)writes s->top from s->nodes: after the traversal nodes or their order
+// words change (build.hip).
+rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
                             uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st);

@@ -11,6 +11,10 @@
 //    first, farther child pushed; traversal stack of child refs in LDS for the
 //    top kStackLds entries ([slot][lane] so lanes hit distinct banks), deeper
 //    entries in a per-thread global spill column sized from the tree depth;
+//  - the first kTopNodes nodes of the binary tree (BFS from the root) are
+//    staged in LDS once per workgroup and read from there: every ray passes
+//    through them, and an LDS read replaces an L2 round trip on the dependent
+//    chain of node fetches;
 //  - conservative slab test: boxes are stored widened outward by a pad of
 //    2^-18 of the scene extent, and the interval is [tmin, t_best] inclusive so
 //    a later, lower-id triangle at the same t is still reached;
@@ -42,11 +46,17 @@ constexpr uint32_t kDry = kMaxParts * kPartStride;
 constexpr uint32_t kArrive = kDry + kMaxParts / 32;
 constexpr uint32_t kCounterWords = kArrive + kPartStride;
 
-// Traversal stack entries per lane held in LDS (deeper entries spill to HBM).
+// Traversal stack entries per lane held in LDS (deeper entries spill to HBM):
+// binary layouts share the workgroup's LDS with the top-of-tree copy, the
+// 8-wide layout (up to 7 pushes per level, no top copy) keeps a deeper one.
 #ifndef RTQ_STACK_LDS
-#define RTQ_STACK_LDS 16
+#define RTQ_STACK_LDS 12
 #endif
-constexpr uint32_t kStackLds = RTQ_STACK_LDS;
+#ifndef RTQ_STACK_LDS_WIDE
+#define RTQ_STACK_LDS_WIDE 16
+#endif
+constexpr uint32_t kStackLdsBinary = RTQ_STACK_LDS;
+constexpr uint32_t kStackLdsWide = RTQ_STACK_LDS_WIDE;
 // Optional VGPR budget of the trace kernels (occupancy is 512 / budget waves
 // per SIMD), or an occupancy the compiler must fit (waves per SIMD).
 #ifdef RTQ_TRACE_VGPRS
@@ -98,6 +108,8 @@ struct TraceArgs {
     uint32_t leaf_slack; // node visits resume once at most this many lanes test triangles
     uint32_t* stat_nodes;
     uint32_t* stat_tris;
+    const float4* top;   // top-of-tree copy (kTopBit refs) of the binary nodes, or null
+    uint32_t n_top;      // its node count (0: traversal starts at global node 0)
 };
 
 __device__ __forceinline__ float sel3(float4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -224,17 +236,23 @@ enum Layout : int {
 // occupancy hint).
 template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
 __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<MULTI>& table) {
+    constexpr uint32_t kStackLds = LAYOUT == kWide ? kStackLdsWide : kStackLdsBinary;
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
+    // the top of the binary tree, read from LDS instead of L2 by every ray
+    constexpr uint32_t kTopSlots = (LAYOUT == kWide || kTopNodes == 0) ? 1u : kTopNodes;
+    __shared__ float4 s_top[4 * kTopSlots];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     uint32_t* const lds = s_stack + tid;  // entry k at lds[k * kTraceThreads]: lanes on distinct banks
     uint32_t* const spill = a.spill + (size_t)blockIdx.x * kTraceThreads + tid;
     const size_t stride = a.spill_stride;
-    if constexpr (MULTI) {
+    const uint32_t n_top = (LAYOUT == kWide || kTopNodes == 0) ? 0u : min(a.n_top, kTopSlots);
+    const uint32_t root = n_top ? kTopBit : 0u;
+    if constexpr (MULTI)
         for (uint32_t i = tid; i < a.n_items; i += kTraceThreads) s_items[i] = table.item[i];
-        __syncthreads();
-    }
+    for (uint32_t i = tid; i < 4 * n_top; i += kTraceThreads) s_top[i] = a.top[i];
+    if (MULTI || n_top) __syncthreads();
     // the lane's scene and ray set (kernel arguments unless batched)
     const float4* nodes_p = a.nodes;
     const float4* tris_p = a.tris;
@@ -372,8 +390,14 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 cur = slot_ref(c03, c47, key[0] & 7u);
             }
         } else {
-            const float4* nd = nodes_p + 4ull * cur;
-            const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+            float4 n0, n1, n2, n3;
+            if (cur & kTopBit) {
+                const float4* nd = s_top + 4 * (cur & ~kTopBit);
+                n0 = nd[0]; n1 = nd[1]; n2 = nd[2]; n3 = nd[3];
+            } else {
+                const float4* nd = nodes_p + 4ull * cur;
+                n0 = nd[0]; n1 = nd[1]; n2 = nd[2]; n3 = nd[3];
+            }
             if (STATS) ++n_nodes;
             const float ax0 = __builtin_fmaf(n0.x, ix, -oix), bx0 = __builtin_fmaf(n0.y, ix, -oix);
             const float ay0 = __builtin_fmaf(n0.z, iy, -oiy), by0 = __builtin_fmaf(n0.w, iy, -oiy);
@@ -500,7 +524,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                     n_nodes = 0;
                     n_tests = 0;
                     sp = 0;
-                    cur = 0;
+                    cur = root;
                     leaf = kEmpty;
                     leaf_k = 0;
                     // box-test reciprocals (zero components replaced by +-1e-30)
@@ -675,6 +699,7 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
     if (blocks == 0) blocks = 1;
     const uint64_t threads = blocks * kTraceThreads;
+    constexpr uint32_t kStackLds = LAYOUT == kWide ? kStackLdsWide : kStackLdsBinary;
     const uint64_t spill_words = depth_entries > kStackLds ? (depth_entries - kStackLds) * threads : 1;
     if (spill_words > sc.spill_words) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -719,6 +744,8 @@ rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, 
     const ItemTable<false> none{};
     const bool wide = layout == kWide;
     a.nodes = wide ? s->wide : s->nodes;
+    a.top = wide ? nullptr : s->top;
+    a.n_top = wide ? 0u : s->n_top;
     TraceScratch& sc = ctx->scratch;
     const uint64_t depth = stack_bound(s, wide);
     if (layout == kWide) return launch<LEAF, MODE, STATS, kWide, false>(ctx, sc, depth, a, none, n, false, st);
@@ -835,6 +862,14 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
     TraceArgs args{};
     args.n = (uint32_t)total;
     args.n_items = n_items;
+    // one top-of-tree copy per launch: when every item traces the same binary nodes
+    const Scene* s0 = find_scene(ctx, items[0].scene_id);
+    bool one_tree = layout != kWide;
+    for (uint32_t i = 1; i < n_items && one_tree; ++i) one_tree = table.item[i].nodes == table.item[0].nodes;
+    if (one_tree) {
+        args.top = s0->top;
+        args.n_top = s0->n_top;
+    }
     TraceScratch& sc = scratch ? *scratch : ctx->scratch;
     const uint32_t nt = (uint32_t)total;
     if (leaf_flags) {

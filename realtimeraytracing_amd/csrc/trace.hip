@@ -345,21 +345,25 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             const float by = __builtin_fmaf(h.y, iy, -oiy);
             const float bz = __builtin_fmaf(h.z, iz, -oiz);
             const uint32_t valid = meta >> 24;
+            // near / far planes by the direction's sign: with ix >= 0 the lower
+            // plane is the nearer (q_lo <= q_hi and the FMA is monotonic, so
+            // this equals min / max of the two distances exactly)
+            const bool px = ix >= 0.0f, py = iy >= 0.0f, pz = iz >= 0.0f;
+            const uint32_t nx0 = px ? qx.x : qx.z, nx1 = px ? qx.y : qx.w, fx0 = px ? qx.z : qx.x, fx1 = px ? qx.w : qx.y;
+            const uint32_t ny0 = py ? qy.x : qy.z, ny1 = py ? qy.y : qy.w, fy0 = py ? qy.z : qy.x, fy1 = py ? qy.w : qy.y;
+            const uint32_t nz0 = pz ? qz.x : qz.z, nz1 = pz ? qz.y : qz.w, fz0 = pz ? qz.z : qz.x, fz1 = pz ? qz.w : qz.y;
             uint32_t key[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int w = j >> 2, sh = 8 * (j & 3);
-                const float lx = (float)(((w ? qx.y : qx.x) >> sh) & 0xFFu);
-                const float hx = (float)(((w ? qx.w : qx.z) >> sh) & 0xFFu);
-                const float ly = (float)(((w ? qy.y : qy.x) >> sh) & 0xFFu);
-                const float hy = (float)(((w ? qy.w : qy.z) >> sh) & 0xFFu);
-                const float lz = (float)(((w ? qz.y : qz.x) >> sh) & 0xFFu);
-                const float hz = (float)(((w ? qz.w : qz.z) >> sh) & 0xFFu);
-                const float t0x = __builtin_fmaf(lx, scx, bx), t1x = __builtin_fmaf(hx, scx, bx);
-                const float t0y = __builtin_fmaf(ly, scy, by), t1y = __builtin_fmaf(hy, scy, by);
-                const float t0z = __builtin_fmaf(lz, scz, bz), t1z = __builtin_fmaf(hz, scz, bz);
-                const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-                const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+                const float tnx = __builtin_fmaf((float)(((w ? nx1 : nx0) >> sh) & 0xFFu), scx, bx);
+                const float tfx = __builtin_fmaf((float)(((w ? fx1 : fx0) >> sh) & 0xFFu), scx, bx);
+                const float tny = __builtin_fmaf((float)(((w ? ny1 : ny0) >> sh) & 0xFFu), scy, by);
+                const float tfy = __builtin_fmaf((float)(((w ? fy1 : fy0) >> sh) & 0xFFu), scy, by);
+                const float tnz = __builtin_fmaf((float)(((w ? nz1 : nz0) >> sh) & 0xFFu), scz, bz);
+                const float tfz = __builtin_fmaf((float)(((w ? fz1 : fz0) >> sh) & 0xFFu), scz, bz);
+                const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest));
                 const bool hit = (tn <= tf) && ((valid >> j) & 1u);
                 // order key: entry distance (non-negative floats order as uints), slot in the low bits
                 key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
@@ -664,13 +668,13 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(W
 // Occupancy hints by layout (0 = the compiler's own choice), from timings on
 // the 2M-triangle terrain: the batched binary kernel at 7 waves per SIMD
 // instead of 6 (74 -> 72 VGPRs, no spill), the single-scene 8-wide kernels at
-// RTQ_OCC_WIDE instead of 5-6; the single-scene binary kernels were slower
-// when pushed from 7 to 8 waves.
+// 7 instead of 6 (74 -> 72 VGPRs, no spill); the single-scene binary kernels
+// were slower when pushed from 7 to 8 waves.
 #ifndef RTQ_OCC_BATCH
 #define RTQ_OCC_BATCH 7
 #endif
 #ifndef RTQ_OCC_WIDE
-#define RTQ_OCC_WIDE 6
+#define RTQ_OCC_WIDE 7
 #endif
 template <int LAYOUT, bool MULTI>
 constexpr int occupancy_hint() {

@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--no-uneven-batch", action="store_true", help="skip the 64-scene uneven batch")
     ap.add_argument("--no-shade", action="store_true", help="skip shading / BVH-view timing")
     ap.add_argument("--no-refit", action="store_true", help="skip the scene-edit refit timing")
+    ap.add_argument("--no-cpu-reference-case", action="store_true",
+                    help="skip the 65,536-triangle soup against the CPU reference traversal")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
@@ -393,6 +395,44 @@ def main() -> int:
                    "overlay_Mrays_s": round(n / (ov_ms * 1e-3) / 1e6, 1)}
         del rgba, ov, mats
 
+    # the reference's own CPU case: 65,536-triangle soup, 256 x 256 camera rays,
+    # RefMT closest hit on the GPU against the reference traversal on the CPU
+    cfg0 = None
+    if not args.no_cpu_reference_case:
+        import oracle
+        from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE, RTQ_TRACE_LEAF_REFMT
+        stris, smodels = scenes.soup_scene(65536, seed=0)
+        ssc = ctx.upload(stris, smodels)
+        sinfo = ssc.build(16, args.max_leaf, 0)
+        scam = scenes.pinhole(**scenes.SOUP_CAMERA)
+        sn = scam.width * scam.height
+        srays, shits = devmem.empty(sn, RAY_DTYPE), devmem.empty(sn, HIT_DTYPE)
+        ctx.raygen_pinhole(scam, srays.data_ptr(), sp)
+        torch.cuda.synchronize()
+
+        def soup_step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            ssc.trace(srays.data_ptr(), sn, shits.data_ptr(), RTQ_TRACE_LEAF_REFMT, sp)
+            if ev is not None:
+                ev[1].record(stream)
+                ev[2].record(stream)
+        s_ms, _ = time_steps(soup_step)
+        got = devmem.to_host(shits, HIT_DTYPE)
+        host_rays = devmem.to_host(srays, RAY_DTYPE)
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+        t_c = time.perf_counter()
+        want = oracle.trace_ref(ssc.export_ref(), oracle.world_vertices(stris, smodels), host_rays,
+                                leaf=oracle.LEAF_REFMT, threads=cores)
+        c_s = time.perf_counter() - t_c
+        cfg0 = {"workload": "65,536-triangle soup (seed 0), 256x256 pinhole rays from (0.5, 0.5, -1), RefMT closest hit",
+                "gpu_Mrays_s": round(sn / (s_ms * 1e-3) / 1e6, 1), "gpu_ms": round(s_ms, 4),
+                "cpu_reference_Mrays_s": round(sn / c_s / 1e6, 2), "cpu_cores": cores,
+                "exact_vs_reference_traversal": bool(np.array_equal(got.view(np.uint8), want.view(np.uint8))),
+                "hit_fraction": round(float((got["id"] >= 0).mean()), 4), "build_ms": round(sinfo.build_ms, 3)}
+        ssc.release()
+
     # scene edit: every terrain vertex moved (heights rippled by up to 0.02),
     # then a refit of the kept tree against a full rebuild
     refit = None
@@ -621,6 +661,7 @@ def main() -> int:
                    "uneven_batch": uneven,
                    "shading": shading,
                    "refit": refit,
+                   "cpu_reference_case": cfg0,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
                    "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
                    "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},

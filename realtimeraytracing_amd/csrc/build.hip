@@ -90,38 +90,33 @@ __global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32
 // ------------------------------------------------------------- top of the tree
 
 // One thread: BFS from the root over internal-node refs, at most kTopNodes
-// nodes; their 64-B records are copied with refs into the set rewritten to
-// kTopBit | slot (leaf refs and refs leaving the set unchanged).
+// nodes.  A child's slot is its position in the BFS queue, known when it is
+// appended, so each node is read once and written with its refs into the set
+// already rewritten to kTopBit | slot (leaf refs and refs leaving the set
+// unchanged).
 __global__ void k_top_nodes(const float4* __restrict__ nodes, float4* __restrict__ top,
                             uint32_t* __restrict__ n_top) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t q[kTopNodes > 0 ? kTopNodes : 1];
     uint32_t n = 0;
     q[n++] = 0;
-    for (uint32_t i = 0; i < n && n < kTopNodes; ++i) {
-        const float4 w = nodes[4ull * q[i] + 3];
-        const uint32_t r0 = __float_as_uint(w.x), r1 = __float_as_uint(w.y);
-        if (!(r0 & kLeafBit) && n < kTopNodes) q[n++] = r0;
-        if (!(r1 & kLeafBit) && n < kTopNodes) q[n++] = r1;
-    }
     for (uint32_t i = 0; i < n; ++i) {
         const float4* src = nodes + 4ull * q[i];
-        float4* dst = top + 4ull * i;
-        dst[0] = src[0];
-        dst[1] = src[1];
-        dst[2] = src[2];
+        const float4 a = src[0], b = src[1], c = src[2];
         float4 w = src[3];
         uint32_t r[2] = {__float_as_uint(w.x), __float_as_uint(w.y)};
-        for (int c = 0; c < 2; ++c) {
-            if (r[c] & kLeafBit) continue;
-            for (uint32_t j = 1; j < n; ++j)
-                if (q[j] == r[c]) {
-                    r[c] = kTopBit | j;
-                    break;
-                }
-        }
+        for (int k = 0; k < 2; ++k)
+            if (!(r[k] & kLeafBit) && n < kTopNodes) {
+                q[n] = r[k];
+                r[k] = kTopBit | n;
+                ++n;
+            }
         w.x = __uint_as_float(r[0]);
         w.y = __uint_as_float(r[1]);
+        float4* dst = top + 4ull * i;
+        dst[0] = a;
+        dst[1] = b;
+        dst[2] = c;
         dst[3] = w;
     }
     *n_top = n;
@@ -313,9 +308,14 @@ __global__ __launch_bounds__(kThreads) void k_leaf_init(const rtq_triangle* __re
 
 // plocNearestNeighborSearch, bvh.cpp:193-210: window [i-r, i+r] of the active
 // cluster list staged through LDS; strict '<' keeps the lowest j on ties.
-__global__ __launch_bounds__(kThreads) void k_ploc_nn(const Box6* __restrict__ b_in, uint32_t iter,
-                                                      uint32_t radius, uint32_t* __restrict__ nn) {
+// PLOC state on the device, so passes can be queued without reading the
+// cluster count back: state[0] = clusters this pass, state[1] = clusters
+// made so far (the next merge id).  Kernels are launched for an upper bound.
+__global__ __launch_bounds__(kThreads) void k_ploc_nn(const Box6* __restrict__ b_in,
+                                                      const uint32_t* __restrict__ state, uint32_t radius,
+                                                      uint32_t* __restrict__ nn) {
     __shared__ Box6 s_box[kThreads + 64];
+    const uint32_t iter = state[0];
     const uint32_t base = blockIdx.x * kThreads;
     const uint32_t lo = base > radius ? base - radius : 0u;
     const uint32_t hi = min(iter, base + kThreads + radius);
@@ -338,10 +338,16 @@ __global__ __launch_bounds__(kThreads) void k_ploc_nn(const Box6* __restrict__ b
 
 // Per position: x = 1 if this position starts a merge (mutual pair, lower
 // index), y = 1 if a cluster stays at this position after merging.
-__global__ __launch_bounds__(kThreads) void k_ploc_flags(const uint32_t* __restrict__ nn, uint32_t iter,
+__global__ __launch_bounds__(kThreads) void k_ploc_flags(const uint32_t* __restrict__ nn,
+                                                         const uint32_t* __restrict__ state, uint32_t bound,
                                                          uint2* __restrict__ flags) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= iter) return;
+    const uint32_t iter = state[0];
+    if (i >= bound) return;
+    if (i >= iter) {
+        flags[i] = make_uint2(0u, 0u);  // past this pass's clusters: counted as nothing
+        return;
+    }
     const uint32_t j = nn[i];
     const bool mutual = (j < iter) && (j != i) && nn[j] == i;
     flags[i] = make_uint2((mutual && i < j) ? 1u : 0u, (mutual && j < i) ? 0u : 1u);
@@ -350,12 +356,13 @@ __global__ __launch_bounds__(kThreads) void k_ploc_flags(const uint32_t* __restr
 // plocMerging + plocCompaction (bvh.cpp:150-191): merged cluster ids are
 // n_total + (number of merges at lower positions), deterministic.
 __global__ __launch_bounds__(kThreads) void k_ploc_merge(
-    uint32_t iter, uint32_t n_total, const uint32_t* __restrict__ nn, const uint2* __restrict__ flags,
+    const uint32_t* __restrict__ state, const uint32_t* __restrict__ nn, const uint2* __restrict__ flags,
     const uint2* __restrict__ offs, const uint32_t* __restrict__ c_in, const Box6* __restrict__ b_in,
     uint32_t* __restrict__ c_out, Box6* __restrict__ b_out, rtq_aabb* __restrict__ cbox,
     uint32_t* __restrict__ cleft, uint32_t* __restrict__ cright, uint32_t* __restrict__ cparent,
     uint32_t* __restrict__ ccount, uint32_t* __restrict__ ctri) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t iter = state[0], n_total = state[1];
     if (i >= iter) return;
     const uint2 f = flags[i];
     if (!f.y) return;
@@ -391,6 +398,15 @@ __global__ __launch_bounds__(kThreads) void k_ploc_merge(
 }
 
 // --------------------------------------------------------------- flattening
+
+// End of a pass: log (first merge id, merges, clusters) and move the state on.
+__global__ void k_ploc_advance(const uint2* __restrict__ tot, uint32_t* __restrict__ state,
+                               uint4* __restrict__ pass_log, uint32_t slot) {
+    const uint2 t = *tot;
+    pass_log[slot] = make_uint4(state[1], t.x, state[0], 0u);
+    state[1] += t.x;
+    state[0] = t.y;
+}
 
 __global__ void k_root_init(uint32_t root, uint32_t* pos, uint32_t* fl, uint32_t* dep) {
     pos[root] = 0;
@@ -700,30 +716,52 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     hipLaunchKernelGGL(k_leaf_init, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models,
                        s->sorted_idx, n, s->cbox, s->ctri, s->ccount, c_in, b_in);
     RTQ_HIP(ctx, hipGetLastError());
+    // passes are queued kPlocBatch at a time against a device-side cluster
+    // count; the host reads the count and the pass log once per batch (passes
+    // queued after the last merge find one cluster and change nothing)
+    constexpr uint32_t kPlocBatch = 8;
     std::vector<std::pair<uint32_t, uint32_t>> pass_ranges;
+    uint32_t *state = nullptr;
+    uint4* pass_log = nullptr;
+    uint2* scan_ws = nullptr;
+    RTQ_HIP(ctx, tmp.get(&state, 2));
+    RTQ_HIP(ctx, tmp.get(&pass_log, kPlocBatch));
+    RTQ_HIP(ctx, tmp.get(&scan_ws, std::max<uint64_t>(scan_u32x2_workspace(n), 1)));
+    const uint32_t h_state0[2] = {n, n};
+    RTQ_HIP(ctx, hipMemcpyAsync(state, h_state0, sizeof h_state0, hipMemcpyHostToDevice, st));
     uint32_t iter = n, n_total = n;
     while (iter > 1) {
-        hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(iter)), dim3(kThreads), 0, st, b_in, iter,
-                           prm.search_radius, nn);
-        RTQ_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(k_ploc_flags, dim3(blocks_for(iter)), dim3(kThreads), 0, st, nn, iter, flags);
-        RTQ_HIP(ctx, hipGetLastError());
-        rs = scan_u32x2(ctx, flags, offs, iter, tot, st);
-        if (rs != RTQ_OK) return rs;
-        hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(iter)), dim3(kThreads), 0, st, iter, n_total, nn,
-                           flags, offs, c_in, b_in, c_out, b_out, s->cbox, s->cleft, s->cright, s->cparent,
-                           s->ccount, s->ctri);
-        RTQ_HIP(ctx, hipGetLastError());
-        uint2 h_tot;
-        RTQ_HIP(ctx, hipMemcpyAsync(&h_tot, tot, sizeof(uint2), hipMemcpyDeviceToHost, st));
+        const uint32_t bound = iter;
+        for (uint32_t k = 0; k < kPlocBatch; ++k) {
+            hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(bound)), dim3(kThreads), 0, st, b_in, state,
+                               prm.search_radius, nn);
+            RTQ_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ploc_flags, dim3(blocks_for(bound)), dim3(kThreads), 0, st, nn, state, bound, flags);
+            RTQ_HIP(ctx, hipGetLastError());
+            rs = scan_u32x2_ws(ctx, flags, offs, bound, tot, scan_ws, st);
+            if (rs != RTQ_OK) return rs;
+            hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(bound)), dim3(kThreads), 0, st, state, nn, flags, offs,
+                               c_in, b_in, c_out, b_out, s->cbox, s->cleft, s->cright, s->cparent, s->ccount,
+                               s->ctri);
+            RTQ_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ploc_advance, dim3(1), dim3(1), 0, st, tot, state, pass_log, k);
+            RTQ_HIP(ctx, hipGetLastError());
+            std::swap(c_in, c_out);
+            std::swap(b_in, b_out);
+        }
+        uint4 h_log[kPlocBatch];
+        uint32_t h_state[2];
+        RTQ_HIP(ctx, hipMemcpyAsync(h_log, pass_log, sizeof h_log, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipMemcpyAsync(h_state, state, sizeof h_state, hipMemcpyDeviceToHost, st));
         RTQ_HIP(ctx, hipStreamSynchronize(st));
-        if (h_tot.x == 0)
-            return fail(ctx, RTQ_EINTERNAL, "PLOC pass with %u clusters made no merge", iter);
-        pass_ranges.emplace_back(n_total, h_tot.x);
-        n_total += h_tot.x;
-        iter = h_tot.y;
-        std::swap(c_in, c_out);
-        std::swap(b_in, b_out);
+        for (uint32_t k = 0; k < kPlocBatch; ++k) {
+            const uint4 e = h_log[k];  // (first merge id, merges, clusters at the pass)
+            if (e.z <= 1) break;
+            if (e.y == 0) return fail(ctx, RTQ_EINTERNAL, "PLOC pass with %u clusters made no merge", e.z);
+            pass_ranges.emplace_back(e.x, e.y);
+        }
+        iter = h_state[0];
+        n_total = h_state[1];
     }
     if (n_total != nc) return fail(ctx, RTQ_EINTERNAL, "PLOC produced %u clusters, expected %u", n_total, nc);
     s->ploc_passes = (uint32_t)pass_ranges.size();

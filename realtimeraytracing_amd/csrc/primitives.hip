@@ -134,6 +134,43 @@ rtq_status scan_impl(rtq_ctx* ctx, const T* in, T* out, uint64_t n, T* total, hi
     return s;
 }
 
+// Elements of workspace scan_impl_ws needs for n elements (the tile sums and
+// their offsets at every level).
+template <typename T>
+uint64_t scan_workspace(uint64_t n) {
+    uint64_t total = 0;
+    while (n > (uint64_t)kScanTile) {
+        const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+        total += 2 * nb;
+        n = nb;
+    }
+    return total;
+}
+
+// The same scan with the caller's workspace: no allocation and no
+// synchronisation, so it can be queued back to back.
+template <typename T>
+rtq_status scan_impl_ws(rtq_ctx* ctx, const T* in, T* out, uint64_t n, T* total, T* ws, hipStream_t st) {
+    if (n == 0) return fail(ctx, RTQ_EINTERNAL, "scan_ws: n must be > 0");
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_tile_scan<T>, dim3(1), dim3(kScanThreads), 0, st, in, out, n,
+                           (const T*)nullptr, total);
+        RTQ_HIP(ctx, hipGetLastError());
+        return RTQ_OK;
+    }
+    T* sums = ws;
+    T* offs = ws + nb;
+    hipLaunchKernelGGL(k_tile_reduce<T>, dim3((uint32_t)nb), dim3(kScanThreads), 0, st, in, n, sums);
+    RTQ_HIP(ctx, hipGetLastError());
+    rtq_status s = scan_impl_ws<T>(ctx, sums, offs, nb, nullptr, ws + 2 * nb, st);
+    if (s != RTQ_OK) return s;
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3((uint32_t)nb), dim3(kScanThreads), 0, st, in, out, n,
+                       (const T*)offs, total);
+    RTQ_HIP(ctx, hipGetLastError());
+    return RTQ_OK;
+}
+
 // ---------------------------------------------------------------- radix sort
 
 constexpr int kSortThreads = 256;
@@ -215,6 +252,14 @@ rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uin
                       hipStream_t st) {
     return scan_impl<U2>(ctx, reinterpret_cast<const U2*>(in), reinterpret_cast<U2*>(out), n,
                          reinterpret_cast<U2*>(total), st);
+}
+
+uint64_t scan_u32x2_workspace(uint64_t n) { return scan_workspace<U2>(n); }
+
+rtq_status scan_u32x2_ws(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total, uint2* ws,
+                         hipStream_t st) {
+    return scan_impl_ws<U2>(ctx, reinterpret_cast<const U2*>(in), reinterpret_cast<U2*>(out), n,
+                            reinterpret_cast<U2*>(total), reinterpret_cast<U2*>(ws), st);
 }
 
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,

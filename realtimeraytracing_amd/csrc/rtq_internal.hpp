@@ -150,6 +150,11 @@ rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
 // Exclusive scan of pairs of u32 (component-wise).
 rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total,
                       hipStream_t st);
+// The same with a caller workspace of scan_u32x2_workspace(n) elements: no
+// allocation, no synchronisation (n > 0).
+uint64_t scan_u32x2_workspace(uint64_t n);
+rtq_status scan_u32x2_ws(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total, uint2* ws,
+                         hipStream_t st);
 // Trace entry points (trace.hip).  `scratch` null = the context's own.
 rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,
                  uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st);

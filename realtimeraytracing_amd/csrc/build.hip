@@ -540,23 +540,20 @@ __global__ void k_write_single(const rtq_aabb* __restrict__ cbox, float pad, flo
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
 
 // Shared front half of the build and of rtq_morton_codes: scene box + cube.
+// d_bounds holds 2 boxes followed by 6 floats per partial block (at most
+// kBoundsPartials): kBoundsBoxes boxes of 32 B.
+constexpr uint32_t kBoundsPartials = 1024;
+constexpr uint32_t kBoundsBoxes = 2 + (6 * kBoundsPartials + 7) / 8;
 rtq_status compute_bounds(rtq_ctx* ctx, Scene* s, uint32_t pad_origin, rtq_aabb* d_bounds,
                           hipStream_t st) {
-    const uint32_t nb = std::min<uint32_t>(blocks_for(s->n_tris), 1024u);
-    float* partial = nullptr;
-    RTQ_HIP(ctx, dev_alloc(&partial, 6ull * nb));
+    const uint32_t nb = std::min<uint32_t>(blocks_for(s->n_tris), kBoundsPartials);
+    float* partial = reinterpret_cast<float*>(d_bounds + 2);
     hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(kThreads), 0, st, s->tris_ref, s->models,
                        s->n_tris, partial);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kThreads), 0, st, partial, nb, s->n_tris,
-                           pad_origin, d_bounds);
-        e = hipGetLastError();
-    }
-    hipError_t se = hipStreamSynchronize(st);
-    (void)hipFree(partial);
-    if (e != hipSuccess) return hip_fail(ctx, e, "bounds kernels");
-    if (se != hipSuccess) return hip_fail(ctx, se, "bounds sync");
+    RTQ_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kThreads), 0, st, partial, nb, s->n_tris,
+                       pad_origin, d_bounds);
+    RTQ_HIP(ctx, hipGetLastError());
     return RTQ_OK;
 }
 
@@ -687,7 +684,7 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     uint32_t *tk = nullptr, *tv = nullptr, *c_in = nullptr, *c_out = nullptr, *nn = nullptr;
     Box6 *b_in = nullptr, *b_out = nullptr;
     uint2 *flags = nullptr, *offs = nullptr, *tot = nullptr;
-    RTQ_HIP(ctx, tmp.get(&d_bounds, 2));
+    RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));
     RTQ_HIP(ctx, tmp.get(&tk, n));
     RTQ_HIP(ctx, tmp.get(&tv, n));
     RTQ_HIP(ctx, tmp.get(&c_in, n));
@@ -806,7 +803,7 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
     const uint32_t n = s->n_tris;
     BuildTemps tmp;
     rtq_aabb* d_bounds = nullptr;
-    RTQ_HIP(ctx, tmp.get(&d_bounds, 2));
+    RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));
     hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, s->ctri,
                        n, s->cbox);
     RTQ_HIP(ctx, hipGetLastError());
@@ -825,7 +822,7 @@ rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_c
     BuildTemps tmp;
     rtq_aabb* d_bounds = nullptr;
     uint32_t* codes = nullptr;
-    RTQ_HIP(ctx, tmp.get(&d_bounds, 2));
+    RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));
     RTQ_HIP(ctx, tmp.get(&codes, s->n_tris));
     rtq_status rs = compute_bounds(ctx, s, flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
     if (rs != RTQ_OK) return rs;

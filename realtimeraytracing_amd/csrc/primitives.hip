@@ -268,9 +268,14 @@ rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32
     if (n >= (1ull << 31)) return fail(ctx, RTQ_ECAPACITY, "sort: n must be < 2^31");
     const uint32_t nn = (uint32_t)n;
     const uint32_t n_tiles = (nn + kSortTile - 1) / kSortTile;
+    // digit counts, their offsets and the offsets scan's workspace in one
+    // allocation, so the passes queue without synchronising
+    const uint64_t n_hist = 256ull * n_tiles;
+    const uint64_t ws = scan_workspace<U1>(n_hist);
     uint32_t* hist = nullptr;
-    RTQ_HIP(ctx, dev_alloc(&hist, 2ull * 256 * n_tiles));
-    uint32_t* offs = hist + 256ull * n_tiles;
+    RTQ_HIP(ctx, dev_alloc(&hist, 2ull * n_hist + ws));
+    uint32_t* offs = hist + n_hist;
+    U1* scan_ws = reinterpret_cast<U1*>(offs + n_hist);
     uint32_t *src_k = keys, *src_v = vals, *dst_k = tmp_k, *dst_v = tmp_v;
     rtq_status s = RTQ_OK;
     int passes = 0;
@@ -279,7 +284,8 @@ rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32
                            n_tiles, hist);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { s = hip_fail(ctx, e, "k_digit_hist"); break; }
-        s = scan_u32(ctx, hist, offs, 256ull * n_tiles, nullptr, st);
+        s = scan_impl_ws<U1>(ctx, reinterpret_cast<const U1*>(hist), reinterpret_cast<U1*>(offs), n_hist,
+                             nullptr, scan_ws, st);
         if (s != RTQ_OK) break;
         hipLaunchKernelGGL(k_digit_scatter, dim3(n_tiles), dim3(kSortThreads), 0, st, src_k, src_v, nn,
                            shift, n_tiles, offs, dst_k, dst_v);

@@ -75,6 +75,7 @@ def parse():
     ap.add_argument("--no-uneven-batch", action="store_true", help="skip the 64-scene uneven batch")
     ap.add_argument("--no-shade", action="store_true", help="skip shading / BVH-view timing")
     ap.add_argument("--no-refit", action="store_true", help="skip the scene-edit refit timing")
+    ap.add_argument("--no-build-timing", action="store_true", help="skip the fused / split build timing")
     ap.add_argument("--no-cpu-reference-case", action="store_true",
                     help="skip the 65,536-triangle soup against the CPU reference traversal")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
@@ -207,7 +208,7 @@ def main() -> int:
                                    "the trace path shards by ray ranges with a replicated BVH (see DESIGN.md)"}))
         return 2
     import torch
-    from realtimeraytracing_amd import (RTQ_TRACE_ANY, RTQ_TRACE_BINARY, RTQ_TRACE_LEARNED, Context, devmem,
+    from realtimeraytracing_amd import (RTQ_BUILD_PLOC_SPLIT, RTQ_TRACE_ANY, RTQ_TRACE_BINARY, RTQ_TRACE_LEARNED, Context, devmem,
                                         scenes)
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -432,6 +433,26 @@ def main() -> int:
                 "exact_vs_reference_traversal": bool(np.array_equal(got.view(np.uint8), want.view(np.uint8))),
                 "hit_fraction": round(float((got["id"] >= 0).mean()), 4), "build_ms": round(sinfo.build_ms, 3)}
         ssc.release()
+
+    # build schedules: fused PLOC passes (default) against one launch per step
+    build = None
+    if not args.no_build_timing:
+        times, outs = {}, {}
+        for name, flags in (("split", RTQ_BUILD_PLOC_SPLIT), ("fused", 0)):
+            ms = []
+            for _ in range(3):
+                binfo = wl.sc.build(16, args.max_leaf, flags, node_width=8)
+                ms.append(binfo.build_ms)
+            times[name] = min(ms)
+            outs[name] = (wl.sc.export_clusters(), binfo.ploc_iterations)
+        (cf, pf), (cs, ps) = outs["fused"], outs["split"]
+        same = pf == ps and all(np.array_equal(cf[k], cs[k]) for k in ("triangle_id", "left", "right", "parent")) \
+            and np.array_equal(cf["boxes"].view(np.uint8), cs["boxes"].view(np.uint8))
+        build = {"workload": "terrain 1024x1024 (2,093,058 tris): bounds, Morton codes, sort, PLOC (radius 16), "
+                             "reference node array, binary and 8-wide traversal layouts; best of 3",
+                 "fused_ms": round(times["fused"], 3), "split_ms": round(times["split"], 3),
+                 "fused_vs_split": round(times["split"] / times["fused"], 3), "ploc_passes": pf,
+                 "same_clusters": bool(same)}
 
     # scene edit: every terrain vertex moved (heights rippled by up to 0.02),
     # then a refit of the kept tree against a full rebuild
@@ -660,6 +681,7 @@ def main() -> int:
                    "compressed_8m": compressed_8m,
                    "uneven_batch": uneven,
                    "shading": shading,
+                   "build": build,
                    "refit": refit,
                    "cpu_reference_case": cfg0,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,

@@ -104,6 +104,9 @@ rtq_status rtq_obj_parse(const char* text, uint64_t len, uint32_t model_id, rtq_
                                              n_tris < 65,536, as the reference does by
                                              scanning its zero-padded triangle vector
                                              (bvh.cpp:237, scene.cpp:26-40) */
+#define RTQ_BUILD_PLOC_SPLIT 0x2u         /* PLOC passes as separate neighbour-search, flag,
+                                             scan and merge launches instead of one fused
+                                             launch per pass; the same clusters either way */
 
 typedef struct rtq_build_params {
     uint32_t search_radius;   /* PLOC neighbour window radius (bvh.hpp:66: 16); 1..32 */
@@ -254,7 +257,11 @@ rtq_status rtq_trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n
  * batch's own scratch (so replays allocate nothing and share nothing with
  * other calls), captures, and instantiates; launch replays it on `stream`.
  * Item buffers must stay valid for the batch's lifetime.  Replays of one batch
- * on different streams must not overlap (they share the batch's scratch). */
+ * on different streams must not overlap (they share the batch's scratch).
+ * The graph holds the node and triangle buffers of the builds it was captured
+ * on: a launch returns RTQ_ENOTBUILT once one of its scenes was rebuilt or
+ * released (create a new batch), or edited and not yet refitted (a refit keeps
+ * the buffers, so the batch replays again afterwards). */
 typedef struct rtq_batch rtq_batch;
 rtq_status rtq_batch_create(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, rtq_batch** out);
 rtq_status rtq_batch_launch(rtq_batch* batch, void* stream);

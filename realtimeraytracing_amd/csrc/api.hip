@@ -47,7 +47,28 @@ static void release(T*& p) {
     p = nullptr;
 }
 
+// Before a rebuild: the buffers sized by the tree go; those sized by the
+// triangle count (cluster arrays, sorted codes, reference nodes, traversal
+// triangles) and the fixed-size top table stay for the build to overwrite.
+void reset_build(Scene* s) {
+    release(s->nodes);
+    release(s->node_aux);
+    release(s->wide);
+    s->n_top = 0;
+    s->built = false;
+    s->stale = false;
+    s->generation = 0;
+    s->passes.clear();
+    s->ordered = false;
+    s->order_closest = false;
+    s->order_any = false;
+    s->n_trav = 0;
+    s->n_wide = 0;
+    s->wide_depth = 0;
+}
+
 void free_build(Scene* s) {
+    reset_build(s);
     release(s->cbox);
     release(s->ctri);
     release(s->cleft);
@@ -57,21 +78,28 @@ void free_build(Scene* s) {
     release(s->sorted_idx);
     release(s->sorted_codes);
     release(s->ref_nodes);
-    release(s->nodes);
-    release(s->node_aux);
     release(s->top);
-    s->n_top = 0;
     release(s->tris);
-    release(s->wide);
-    s->built = false;
-    s->stale = false;
-    s->passes.clear();
-    s->ordered = false;
-    s->order_closest = false;
-    s->order_any = false;
-    s->n_trav = 0;
-    s->n_wide = 0;
-    s->wide_depth = 0;
+}
+
+rtq_status arena_reserve(rtq_ctx* ctx) {
+    Arena& a = ctx->arena;
+    if (a.top != 0 || a.extra != 0) return fail(ctx, RTQ_EINTERNAL, "arena in use at the start of an operation");
+    if (a.peak <= a.cap) return RTQ_OK;
+    if (a.base) {
+        (void)hipFree(a.base);
+        a.base = nullptr;
+        a.cap = 0;
+    }
+    const size_t want = (a.peak + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&a.base), want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        a.base = nullptr;  // temporaries fall back to their own allocations
+        return RTQ_OK;
+    }
+    a.cap = want;
+    return RTQ_OK;
 }
 
 static void free_scene(Scene* s) {
@@ -302,6 +330,7 @@ void rtq_destroy(rtq_ctx* ctx) {
     for (auto& kv : ctx->scenes) free_scene(kv.second);
     ctx->scenes.clear();
     scratch_free(&ctx->scratch);
+    if (ctx->arena.base) (void)hipFree(ctx->arena.base);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -452,7 +481,8 @@ rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params
     if (params) p = *params;
     if (p.search_radius < 1 || p.search_radius > 32) return fail(ctx, RTQ_EINVAL, "bvh_build: search_radius must be 1..32");
     if (p.max_leaf_size < 1 || p.max_leaf_size > kMaxLeaf) return fail(ctx, RTQ_EINVAL, "bvh_build: max_leaf_size must be 1..8");
-    if (p.flags & ~RTQ_BUILD_REF_PADDED_BOUNDS) return fail(ctx, RTQ_EINVAL, "bvh_build: unknown flags 0x%x", p.flags);
+    if (p.flags & ~(RTQ_BUILD_REF_PADDED_BOUNDS | RTQ_BUILD_PLOC_SPLIT))
+        return fail(ctx, RTQ_EINVAL, "bvh_build: unknown flags 0x%x", p.flags);
     if (p.node_width == 0) p.node_width = 2;
     if (p.node_width != 2 && p.node_width != 8) return fail(ctx, RTQ_EINVAL, "bvh_build: node_width must be 2 or 8");
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
@@ -553,13 +583,20 @@ rtq_status rtq_sort_pairs(rtq_ctx* ctx, uint32_t* keys_dev, uint32_t* vals_dev, 
     if (!keys_dev || !vals_dev) return fail(ctx, RTQ_EINVAL, "sort_pairs: null pointer");
     if (key_bits > 32) return fail(ctx, RTQ_EINVAL, "sort_pairs: key_bits must be <= 32");
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
-    uint32_t *tk = nullptr, *tv = nullptr;
-    RTQ_HIP(ctx, dev_alloc(&tk, n));
-    hipError_t e = dev_alloc(&tv, n);
-    if (e != hipSuccess) { (void)hipFree(tk); return hip_fail(ctx, e, "sort_pairs temp"); }
-    rtq_status s = radix_sort_pairs(ctx, keys_dev, vals_dev, tk, tv, n, key_bits, pick_stream(ctx, stream));
-    (void)hipFree(tk);
-    (void)hipFree(tv);
+    RTQ_HIP(ctx, hipDeviceSynchronize());  // nothing queued still uses the arena
+    rtq_status s = arena_reserve(ctx);
+    if (s != RTQ_OK) return s;
+    const hipStream_t st = pick_stream(ctx, stream);
+    {
+        Temps tmp(ctx);
+        uint32_t *tk = nullptr, *tv = nullptr;
+        RTQ_HIP(ctx, tmp.get(&tk, n));
+        RTQ_HIP(ctx, tmp.get(&tv, n));
+        s = radix_sort_pairs(ctx, keys_dev, vals_dev, tk, tv, n, key_bits, st);
+        // the temporaries go back to the arena once the sort is done
+        const hipError_t e = hipStreamSynchronize(st);
+        if (s == RTQ_OK && e != hipSuccess) s = hip_fail(ctx, e, "hipStreamSynchronize");
+    }
     return s;
 }
 
@@ -568,7 +605,14 @@ rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t
     if (n == 0) return RTQ_OK;
     if (!in_dev || !out_dev) return fail(ctx, RTQ_EINVAL, "scan: null pointer");
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
-    return scan_u32(ctx, in_dev, out_dev, n, nullptr, pick_stream(ctx, stream));
+    RTQ_HIP(ctx, hipDeviceSynchronize());  // nothing queued still uses the arena
+    rtq_status s = arena_reserve(ctx);
+    if (s != RTQ_OK) return s;
+    const hipStream_t st = pick_stream(ctx, stream);
+    s = scan_u32(ctx, in_dev, out_dev, n, nullptr, st);
+    const hipError_t e = hipStreamSynchronize(st);  // its temporaries go back to the arena
+    if (s == RTQ_OK && e != hipSuccess) s = hip_fail(ctx, e, "hipStreamSynchronize");
+    return s;
 }
 
 static rtq_status trace_common(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays, uint64_t n, uint32_t flags,

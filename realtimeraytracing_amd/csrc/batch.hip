@@ -107,6 +107,7 @@ rtq_status rtq_batch_create(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t 
     if (e != hipSuccess) return bail(hip_fail(ctx, e, "hipStreamEndCapture"));
     e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) return bail(hip_fail(ctx, e, "hipGraphInstantiate"));
+    for (uint32_t i = 0; i < n_items; ++i) b->generations.push_back(find_scene(ctx, items[i].scene_id)->generation);
     *out = b;
     return RTQ_OK;
 }
@@ -114,6 +115,17 @@ rtq_status rtq_batch_create(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t 
 rtq_status rtq_batch_launch(rtq_batch* b, void* stream) {
     if (!b || !b->exec) return RTQ_EINVAL;
     rtq_ctx* ctx = b->ctx;
+    // the graph holds the node, triangle and spill buffers of the builds it
+    // was captured on: a rebuild or release frees them, an edit without a
+    // refit leaves the boxes behind the vertices
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        const Scene* s = find_scene(ctx, b->items[i].scene_id);
+        if (s && s->stale && s->generation == b->generations[i])
+            return fail(ctx, RTQ_ENOTBUILT, "batch_launch: scene %u was edited and not refitted", b->items[i].scene_id);
+        if (!s || !s->built || s->generation != b->generations[i])
+            return fail(ctx, RTQ_ENOTBUILT, "batch_launch: scene %u was rebuilt or released after the batch was created",
+                        b->items[i].scene_id);
+    }
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
     RTQ_HIP(ctx, hipGraphLaunch(b->exec, pick_stream(ctx, stream)));
     return RTQ_OK;

@@ -408,6 +408,390 @@ __global__ void k_ploc_advance(const uint2* __restrict__ tot, uint32_t* __restri
     state[0] = t.y;
 }
 
+// ------------------------------------------------------- fused PLOC passes
+//
+// The single-launch PLOC iteration of PLOC++ (Benthin, Meister, Barczak,
+// Mohanty & Wald, HPG 2022), replacing the neighbour search, flag, scan (two
+// or three launches), merge and state-update launches of a pass.  Workgroup w
+// owns the T = kPassThreads - 2r positions [wT, wT + T) of the active list. It
+// stages the boxes of [wT - 2r, wT + T + 2r) in LDS and finds the nearest
+// neighbour of every position of [wT - r, wT + T + r), so the partner of each
+// owned position has its own neighbour at hand.  It then sets the merge and
+// stay flags of its positions, scans them inside the workgroup and gets the
+// counts before its tile by decoupled look-back over per-tile status words
+// (Merrill & Garland, "Single-pass Parallel Prefix Scan with Decoupled
+// Look-back", 2016).  Tiles are numbered in the order workgroups start (an
+// atomic counter), so every tile a workgroup waits on is already running, and
+// the wait is bounded (kSpinLimit) so a broken invariant ends in an error
+// flag instead of a hang.  A status word carries its counts, so it is stored
+// and loaded with relaxed device-scope atomics: acquire / release would add an
+// L2 write-back or invalidate per poll.  A tile whose wait gave up publishes
+// kTilePoison; a tile whose look-back reaches one poisons itself too and
+// neither writes its clusters.  Each position goes through the arithmetic of
+// k_ploc_nn, k_ploc_flags and k_ploc_merge, so the clusters are identical.
+//
+// Passes with at most kTailMax clusters (33 of the 61 passes of a 2M-triangle
+// terrain) run in one workgroup, k_ploc_tail, the whole list held in LDS.
+//
+// Device state per pass slot: {clusters, next merge id, list buffer 0/1, 0};
+// a pass reads slot k & 1 and writes slot (k + 1) & 1, so no workgroup reads
+// a word another one writes.
+
+constexpr int kPassThreads = 512;
+constexpr int kTailThreads = 1024;
+constexpr uint32_t kTailMax = 4 * kTailThreads;   // 4 positions per tail thread
+constexpr uint64_t kTileAgg = 1ull << 62;         // tile status: own counts published
+constexpr uint64_t kTilePre = 2ull << 62;         // tile status: counts through this tile
+constexpr uint64_t kTilePoison = 3ull << 62;      // tile status: a wait at or before it gave up
+constexpr uint64_t kTileState = 3ull << 62;
+constexpr uint32_t kSpinLimit = 1u << 24;         // ~0.5 s of s_sleep(1) polls
+
+enum : uint32_t { kPlocErrSpin = 1, kPlocErrNoNeighbour = 2, kPlocErrBound = 4, kPlocErrNoMerge = 8 };
+
+struct PlocLists {
+    uint32_t* c[2];
+    Box6* b[2];
+};
+
+struct PlocClusters {
+    rtq_aabb* box;
+    uint32_t *left, *right, *parent, *count, *tri;
+};
+
+__device__ __forceinline__ uint64_t tile_word(uint64_t st, uint32_t merges, uint32_t stays) {
+    return st | ((uint64_t)merges << 32) | stays;
+}
+
+// Exclusive sum over the workgroup; `total` is the workgroup's sum.  Every
+// thread calls it; s_w holds one word per wave.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* s_w, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / 64; ++w) {
+        const uint32_t x = s_w[w];
+        before += (w < wave) ? x : 0u;
+        all += x;
+    }
+    total = all;
+    return before + incl - v;
+}
+
+// Merge of clusters cid (box b) and cj (box bj) into cluster id, as
+// k_ploc_merge writes it; b becomes the merged box.
+__device__ __forceinline__ void ploc_merge_write(const PlocClusters& cl, uint32_t id, uint32_t cid, uint32_t cj,
+                                                 Box6& b, const Box6& bj) {
+    Box6 m;
+    rtq_aabb a{};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        m.lo[k] = smin(b.lo[k], bj.lo[k]);
+        m.hi[k] = smax(b.hi[k], bj.hi[k]);
+        a.min[k] = m.lo[k];
+        a.max[k] = m.hi[k];
+    }
+    cl.box[id] = a;
+    cl.left[id] = cid;
+    cl.right[id] = cj;
+    cl.parent[cid] = id;
+    cl.parent[cj] = id;
+    cl.count[id] = cl.count[cid] + cl.count[cj];
+    cl.tri[id] = 0;
+    b = m;
+}
+
+struct PlocPassArgs {
+    const uint32_t* st_in;
+    uint32_t* st_out;
+    uint32_t bound;       // clusters at the start of the batch: the grid covers them
+    uint32_t radius;
+    uint32_t* tile_ctr;   // zeroed before the pass
+    uint64_t* tile_status;// one word per tile, zeroed before the pass
+    uint32_t* err;
+    uint4* log;           // (first merge id, merges, clusters, 1 if left to the tail)
+    PlocLists lists;
+    PlocClusters cl;
+};
+
+__global__ __launch_bounds__(kPassThreads) void k_ploc_pass(PlocPassArgs a) {
+    __shared__ Box6 s_box[kPassThreads + 64];
+    __shared__ uint32_t s_nn[kPassThreads];
+    __shared__ uint32_t s_w[kPassThreads / 64];
+    __shared__ uint32_t s_hdr[5];  // tile, clusters, next merge id, list buffer, skip
+    __shared__ uint32_t s_excl[3]; // merges and stays before the tile, abort
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = (int)a.radius;
+    const uint32_t T = (uint32_t)(kPassThreads - 2 * r);
+    if (tid == 0) {
+        const uint32_t tile = atomicAdd(a.tile_ctr, 1u);
+        const uint32_t iter = a.st_in[0], n_total = a.st_in[1], cur = a.st_in[2] & 1u;
+        uint32_t skip = 0;
+        if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || iter <= kTailMax) {
+            skip = 1;
+        } else if (iter > a.bound) {
+            atomicOr(a.err, kPlocErrBound);
+            skip = 1;
+        }
+        if (skip && tile == 0) {  // left to the tail kernel (or stopped): carry the state over
+            a.st_out[0] = iter;
+            a.st_out[1] = n_total;
+            a.st_out[2] = cur;
+            *a.log = make_uint4(n_total, 0u, iter, 1u);
+        }
+        if (!skip && (uint64_t)tile * T >= iter) skip = 1;
+        s_hdr[0] = tile;
+        s_hdr[1] = iter;
+        s_hdr[2] = n_total;
+        s_hdr[3] = cur;
+        s_hdr[4] = skip;
+    }
+    __syncthreads();
+    if (s_hdr[4]) return;
+    const uint32_t tile = s_hdr[0], iter = s_hdr[1], n_total = s_hdr[2], cur = s_hdr[3];
+    const uint32_t* __restrict__ c_in = a.lists.c[cur];
+    const Box6* __restrict__ b_in = a.lists.b[cur];
+    const int base = (int)(tile * T);
+    const int lo = base - 2 * r;
+    for (int q = tid; q < kPassThreads + 2 * r; q += kPassThreads) {
+        const int p = lo + q;
+        if (p >= 0 && p < (int)iter) s_box[q] = b_in[p];
+    }
+    __syncthreads();
+
+    // nearest neighbour of position i = base - r + tid (k_ploc_nn); its box
+    // is s_box[tid + r] and the window [i - r, i + r] is s_box[tid .. tid + 2r]
+    const int i = base - r + tid;
+    {
+        uint32_t arg = kNone;
+        if (i >= 0 && i < (int)iter) {
+            const Box6 bi = s_box[tid + r];
+            const int start = max(i - r, 0), end = min(i + r + 1, (int)iter);
+            float best = INFINITY;
+            for (int j = start; j < end; ++j) {
+                if (j == i) continue;
+                const float d = merged_sa(bi, s_box[j - lo]);
+                if (d < best) { best = d; arg = (uint32_t)j; }
+            }
+            if (arg == kNone) atomicOr(a.err, kPlocErrNoNeighbour);
+        }
+        s_nn[tid] = arg;
+    }
+    __syncthreads();
+
+    // flags of the owned positions (k_ploc_flags): the partner j lies in
+    // [i - r, i + r], so its neighbour is s_nn[tid + (j - i)]
+    const bool owned = tid >= r && tid < r + (int)T && i < (int)iter;
+    uint32_t fx = 0, fy = 0, j = kNone;
+    if (owned) {
+        j = s_nn[tid];
+        const bool mutual = j != kNone && s_nn[tid + ((int)j - i)] == (uint32_t)i;
+        fx = (mutual && (uint32_t)i < j) ? 1u : 0u;
+        fy = (mutual && j < (uint32_t)i) ? 0u : 1u;
+    }
+    uint32_t total = 0;
+    const uint32_t v = (fx << 16) | fy;  // at most kPassThreads of each per tile
+    const uint32_t local = block_exclusive_sum<kPassThreads>(v, s_w, total);
+
+    // counts before this tile: decoupled look-back by wave 0
+    if (wave == 0) {
+        const uint32_t ax = total >> 16, ay = total & 0xFFFFu;
+        uint32_t ex = 0, ey = 0;
+        bool failed = false;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&a.tile_status[0], tile_word(kTilePre, ax, ay), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&a.tile_status[tile], tile_word(kTileAgg, ax, ay), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int64_t top = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                const int64_t idx = top - lane;  // lane 0 = nearest tile
+                uint64_t w = kTilePre;           // before tile 0: an empty prefix
+                if (idx >= 0)
+                    w = __hip_atomic_load(&a.tile_status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (__ballot((w & kTileState) == 0) != 0) {
+                    if (++spins > kSpinLimit) {
+                        failed = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((w & kTileState) == 0)
+                        w = __hip_atomic_load(&a.tile_status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (failed) break;
+                // nearest tile with counts through it (a prefix, or poison)
+                const unsigned long long pre = __ballot((w & kTileState) >= kTilePre);
+                const int stop = pre ? __ffsll((long long)pre) - 1 : 63;
+                if (pre && __shfl((uint32_t)(w >> 62), stop, 64) == 3u) {
+                    failed = true;
+                    break;
+                }
+                uint32_t cx = lane <= stop ? (uint32_t)(w >> 32) & 0x3FFFFFFFu : 0u;
+                uint32_t cy = lane <= stop ? (uint32_t)w : 0u;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) {
+                    cx += __shfl_xor(cx, d, 64);
+                    cy += __shfl_xor(cy, d, 64);
+                }
+                ex += cx;
+                ey += cy;
+                if (pre) break;
+                top -= 64;
+            }
+            if (lane == 0) {
+                if (failed) atomicOr(a.err, kPlocErrSpin);
+                __hip_atomic_store(&a.tile_status[tile],
+                                   failed ? kTilePoison : tile_word(kTilePre, ex + ax, ey + ay), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (lane == 0) {
+            s_excl[0] = ex;
+            s_excl[1] = ey;
+            s_excl[2] = failed ? 1u : 0u;
+            if (!failed && tile == (iter - 1) / T) {  // the tile holding the last position ends the pass
+                a.st_out[0] = ey + ay;
+                a.st_out[1] = n_total + ex + ax;
+                a.st_out[2] = cur ^ 1u;
+                *a.log = make_uint4(n_total, ex + ax, iter, 0u);
+            }
+        }
+    }
+    __syncthreads();
+    if (s_excl[2] || !owned || !fy) return;
+
+    // merge and compaction (k_ploc_merge)
+    const uint32_t ox = s_excl[0] + (local >> 16), oy = s_excl[1] + (local & 0xFFFFu);
+    uint32_t cid = c_in[i];
+    Box6 b = s_box[tid + r];
+    if (fx) {
+        ploc_merge_write(a.cl, n_total + ox, cid, c_in[j], b, s_box[(int)j - lo]);
+        cid = n_total + ox;
+    }
+    a.lists.c[cur ^ 1u][oy] = cid;
+    a.lists.b[cur ^ 1u][oy] = b;
+}
+
+// All remaining passes once at most kTailMax clusters are left, in one
+// workgroup with the list in LDS (same arithmetic as k_ploc_pass).  Thread t
+// owns positions 4t .. 4t+3 for flags, scan and merge.  Does nothing unless
+// 1 < clusters <= kTailMax.
+__global__ __launch_bounds__(kTailThreads) void k_ploc_tail(uint32_t* __restrict__ state, uint32_t radius,
+                                                            uint32_t* __restrict__ err, uint4* __restrict__ log,
+                                                            uint32_t* __restrict__ log_count, PlocLists lists,
+                                                            PlocClusters cl) {
+    __shared__ Box6 s_box[kTailMax];
+    __shared__ uint32_t s_c[kTailMax];
+    __shared__ uint16_t s_nn[kTailMax];
+    __shared__ uint32_t s_w[kTailThreads / 64];
+    __shared__ uint32_t s_hdr[4];
+    constexpr uint16_t kNone16 = 0xFFFFu;
+    const int tid = threadIdx.x;
+    const int r = (int)radius;
+    if (tid == 0) {
+        const uint32_t m0 = state[0];
+        s_hdr[0] = m0;
+        s_hdr[1] = state[1];
+        s_hdr[2] = state[2] & 1u;
+        s_hdr[3] = (*err != 0 || m0 <= 1 || m0 > kTailMax) ? 1u : 0u;
+        if (s_hdr[3]) *log_count = 0;
+    }
+    __syncthreads();
+    if (s_hdr[3]) return;
+    uint32_t m = s_hdr[0], n_total = s_hdr[1];
+    const uint32_t cur = s_hdr[2];
+    for (uint32_t q = tid; q < m; q += kTailThreads) {
+        s_box[q] = lists.b[cur][q];
+        s_c[q] = lists.c[cur][q];
+    }
+    __syncthreads();
+    uint32_t passes = 0;
+    while (m > 1) {
+        for (int i = tid; i < (int)m; i += kTailThreads) {
+            const Box6 bi = s_box[i];
+            const int start = max(i - r, 0), end = min(i + r + 1, (int)m);
+            float best = INFINITY;
+            uint16_t arg = kNone16;
+            for (int j = start; j < end; ++j) {
+                if (j == i) continue;
+                const float d = merged_sa(bi, s_box[j]);
+                if (d < best) { best = d; arg = (uint16_t)j; }
+            }
+            if (arg == kNone16) atomicOr(err, kPlocErrNoNeighbour);
+            s_nn[i] = arg;
+        }
+        __syncthreads();
+        uint32_t f[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = 4u * tid + k;
+            f[k] = 0;
+            if (i < m) {
+                const uint32_t j = s_nn[i];
+                const bool mutual = j != kNone16 && s_nn[j] == i;
+                f[k] = ((mutual && i < j) ? 1u << 16 : 0u) | ((mutual && j < i) ? 0u : 1u);
+            }
+            sum += f[k];
+        }
+        uint32_t total = 0;
+        uint32_t run = block_exclusive_sum<kTailThreads>(sum, s_w, total);
+        Box6 ob[4];
+        uint32_t oc[4], opos[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = 4u * tid + k;
+            opos[k] = kNone;
+            if (i < m && (f[k] & 0xFFFFu)) {
+                uint32_t cid = s_c[i];
+                Box6 b = s_box[i];
+                if (f[k] >> 16) {
+                    const uint32_t j = s_nn[i], id = n_total + (run >> 16);
+                    ploc_merge_write(cl, id, cid, s_c[j], b, s_box[j]);
+                    cid = id;
+                }
+                ob[k] = b;
+                oc[k] = cid;
+                opos[k] = run & 0xFFFFu;
+            }
+            run += f[k];
+        }
+        __syncthreads();  // every read of this pass's list is done
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (opos[k] != kNone) {
+                s_box[opos[k]] = ob[k];
+                s_c[opos[k]] = oc[k];
+            }
+        const uint32_t merges = total >> 16, stays = total & 0xFFFFu;
+        if (tid == 0) log[passes] = make_uint4(n_total, merges, m, 0u);
+        if (merges == 0) {  // uniform: no progress, the build reports it
+            if (tid == 0) atomicOr(err, kPlocErrNoMerge);
+            break;
+        }
+        n_total += merges;
+        m = stays;
+        ++passes;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        state[0] = m;
+        state[1] = n_total;
+        *log_count = passes;
+    }
+}
+
 __global__ void k_root_init(uint32_t root, uint32_t* pos, uint32_t* fl, uint32_t* dep) {
     pos[root] = 0;
     fl[root] = 0;
@@ -447,25 +831,31 @@ __global__ __launch_bounds__(kThreads) void k_write_ref(
     const float4* __restrict__ tri_world, rtq_bvh_node* __restrict__ ref, float4* __restrict__ tris,
     uint32_t* __restrict__ max_depth) {
     const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
-    if (c >= n_clusters) return;
-    rtq_bvh_node nd{};
-    nd.box = cbox[c];
-    nd.box.pad0_ = 0.0f;
-    nd.box.pad1_ = 0.0f;
-    const bool leaf = cleft[c] == kNone;
-    nd.triangle_id = leaf ? ctri[c] : 0u;
-    nd.left_child = leaf ? 0u : pos[cleft[c]];
-    nd.right_child = leaf ? 0u : pos[cright[c]];
-    nd.pad_ = 0u;
-    ref[pos[c]] = nd;
-    if (leaf) {
-        const uint32_t t = ctri[c];
-        const uint64_t dst = 3ull * fl[c];
-        tris[dst + 0] = tri_world[3ull * t + 0];
-        tris[dst + 1] = tri_world[3ull * t + 1];
-        tris[dst + 2] = tri_world[3ull * t + 2];
-        atomicMax(max_depth, dep[c]);
+    uint32_t depth = 0;
+    if (c < n_clusters) {
+        rtq_bvh_node nd{};
+        nd.box = cbox[c];
+        nd.box.pad0_ = 0.0f;
+        nd.box.pad1_ = 0.0f;
+        const bool leaf = cleft[c] == kNone;
+        nd.triangle_id = leaf ? ctri[c] : 0u;
+        nd.left_child = leaf ? 0u : pos[cleft[c]];
+        nd.right_child = leaf ? 0u : pos[cright[c]];
+        nd.pad_ = 0u;
+        ref[pos[c]] = nd;
+        if (leaf) {
+            const uint32_t t = ctri[c];
+            const uint64_t dst = 3ull * fl[c];
+            tris[dst + 0] = tri_world[3ull * t + 0];
+            tris[dst + 1] = tri_world[3ull * t + 1];
+            tris[dst + 2] = tri_world[3ull * t + 2];
+            depth = dep[c];
+        }
     }
+    // deepest leaf: one atomic per wave instead of one per leaf
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) depth = max(depth, (uint32_t)__shfl_xor(depth, d, 64));
+    if ((threadIdx.x & 63) == 0 && depth > 0) atomicMax(max_depth, depth);
 }
 
 __device__ __forceinline__ bool is_trav_node(uint32_t c, uint32_t root, uint32_t max_leaf,
@@ -557,18 +947,145 @@ rtq_status compute_bounds(rtq_ctx* ctx, Scene* s, uint32_t pad_origin, rtq_aabb*
     return RTQ_OK;
 }
 
-struct BuildTemps {
-    std::vector<void*> ptrs;
-    ~BuildTemps() {
-        for (void* p : ptrs) (void)hipFree(p);
+
+// PLOC with fused passes: batches of kPlocBatch k_ploc_pass launches (each a
+// no-op once at most kTailMax clusters are left) and one k_ploc_tail launch
+// (a no-op until then), then one read of the state and the pass logs.  The
+// active list starts in (c0, b0); c1, b1 are the other buffers.
+rtq_status ploc_fused(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint32_t* c0, uint32_t* c1,
+                      Box6* b0, Box6* b1, std::vector<std::pair<uint32_t, uint32_t>>& passes, hipStream_t st) {
+    constexpr uint32_t kPlocBatch = 8;
+    const uint32_t n = s->n_tris;
+    const uint32_t T = kPassThreads - 2 * radius;
+    const uint32_t max_tiles = (n + T - 1) / T;
+    // ctl: kPlocBatch tile counters, then kPlocBatch x max_tiles status words
+    const size_t ctl_words = 2 * kPlocBatch + (size_t)kPlocBatch * max_tiles * 2;
+    uint32_t *ctl = nullptr, *state = nullptr, *err = nullptr, *tail_count = nullptr;
+    uint4 *log = nullptr, *tail_log = nullptr;
+    RTQ_HIP(ctx, tmp.get(&ctl, ctl_words));
+    RTQ_HIP(ctx, tmp.get(&state, 8));
+    RTQ_HIP(ctx, tmp.get(&err, 2));
+    RTQ_HIP(ctx, tmp.get(&log, kPlocBatch));
+    RTQ_HIP(ctx, tmp.get(&tail_log, kTailMax));
+    tail_count = err + 1;
+    const uint32_t h_state0[8] = {n, n, 0u, 0u, 0u, 0u, 0u, 0u};
+    RTQ_HIP(ctx, hipMemcpyAsync(state, h_state0, sizeof h_state0, hipMemcpyHostToDevice, st));
+    RTQ_HIP(ctx, hipMemsetAsync(err, 0, 2 * sizeof(uint32_t), st));
+    PlocLists lists{{c0, c1}, {b0, b1}};
+    PlocClusters cl{s->cbox, s->cleft, s->cright, s->cparent, s->ccount, s->ctri};
+    uint32_t iter = n, n_total = n;
+    std::vector<uint4> h_tail;
+    while (iter > 1) {
+        const uint32_t bound = iter;
+        if (iter > kTailMax) {
+            const uint32_t tiles = (bound + T - 1) / T;
+            RTQ_HIP(ctx, hipMemsetAsync(ctl, 0, sizeof(uint32_t) * (2ull * kPlocBatch + 2ull * kPlocBatch * tiles), st));
+            uint64_t* status = reinterpret_cast<uint64_t*>(ctl + 2 * kPlocBatch);
+            for (uint32_t k = 0; k < kPlocBatch; ++k) {
+                PlocPassArgs a{state + 4 * (k & 1), state + 4 * ((k + 1) & 1), bound, radius, ctl + k,
+                               status + (size_t)k * tiles, err, log + k, lists, cl};
+                hipLaunchKernelGGL(k_ploc_pass, dim3(tiles), dim3(kPassThreads), 0, st, a);
+                RTQ_HIP(ctx, hipGetLastError());
+            }
+        }
+        // the state of the batch's last pass is in slot 0 (kPlocBatch is even)
+        hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(kTailThreads), 0, st, state, radius, err, tail_log, tail_count,
+                           lists, cl);
+        RTQ_HIP(ctx, hipGetLastError());
+        uint4 h_log[kPlocBatch];
+        uint32_t h_state[4], h_err[2];
+        RTQ_HIP(ctx, hipMemcpyAsync(h_log, log, sizeof h_log, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipMemcpyAsync(h_state, state, sizeof h_state, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipMemcpyAsync(h_err, err, sizeof h_err, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        if (h_err[0] & kPlocErrNoMerge) return fail(ctx, RTQ_EINTERNAL, "PLOC pass made no merge");
+        if (h_err[0])
+            return fail(ctx, RTQ_EINTERNAL, "fused PLOC pass failed (error bits 0x%x)", h_err[0]);
+        if (iter > kTailMax)
+            for (uint32_t k = 0; k < kPlocBatch; ++k) {
+                const uint4 e = h_log[k];  // (first merge id, merges, clusters at the pass, left to the tail)
+                if (e.w) break;
+                if (e.y == 0) return fail(ctx, RTQ_EINTERNAL, "PLOC pass with %u clusters made no merge", e.z);
+                passes.emplace_back(e.x, e.y);
+            }
+        const uint32_t nt = h_err[1];
+        if (nt > kTailMax) return fail(ctx, RTQ_EINTERNAL, "PLOC tail ran %u passes", nt);
+        if (nt) {
+            h_tail.resize(nt);
+            RTQ_HIP(ctx, hipMemcpyAsync(h_tail.data(), tail_log, sizeof(uint4) * nt, hipMemcpyDeviceToHost, st));
+            RTQ_HIP(ctx, hipStreamSynchronize(st));
+            for (const uint4& e : h_tail) passes.emplace_back(e.x, e.y);
+        }
+        if (h_state[0] >= iter && h_state[0] > 1)
+            return fail(ctx, RTQ_EINTERNAL, "PLOC made no progress at %u clusters", iter);
+        iter = h_state[0];
+        n_total = h_state[1];
     }
-    template <typename T>
-    hipError_t get(T** p, size_t count) {
-        hipError_t e = dev_alloc(p, count);
-        if (e == hipSuccess) ptrs.push_back(*p);
-        return e;
+    if (n_total != 2 * n - 1) return fail(ctx, RTQ_EINTERNAL, "PLOC produced %u clusters, expected %u", n_total, 2 * n - 1);
+    return RTQ_OK;
+}
+
+// PLOC with one launch per step of a pass: neighbour search, flags, scan,
+// merge and state update (RTQ_BUILD_PLOC_SPLIT).  Passes are queued kPlocBatch
+// at a time against a device-side cluster count; the host reads the count and
+// the pass log once per batch (passes queued after the last merge find one
+// cluster and change nothing).
+rtq_status ploc_split(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint32_t* c_in, uint32_t* c_out,
+                      Box6* b_in, Box6* b_out, std::vector<std::pair<uint32_t, uint32_t>>& pass_ranges,
+                      hipStream_t st) {
+    constexpr uint32_t kPlocBatch = 8;
+    const uint32_t n = s->n_tris;
+    uint32_t *nn = nullptr, *state = nullptr;
+    uint2 *flags = nullptr, *offs = nullptr, *tot = nullptr, *scan_ws = nullptr;
+    uint4* pass_log = nullptr;
+    RTQ_HIP(ctx, tmp.get(&nn, n));
+    RTQ_HIP(ctx, tmp.get(&flags, n));
+    RTQ_HIP(ctx, tmp.get(&offs, n));
+    RTQ_HIP(ctx, tmp.get(&tot, 1));
+    RTQ_HIP(ctx, tmp.get(&state, 2));
+    RTQ_HIP(ctx, tmp.get(&pass_log, kPlocBatch));
+    RTQ_HIP(ctx, tmp.get(&scan_ws, std::max<uint64_t>(scan_u32x2_workspace(n), 1)));
+    RTQ_HIP(ctx, hipMemsetAsync(nn, 0, sizeof(uint32_t) * n, st));
+    const uint32_t h_state0[2] = {n, n};
+    RTQ_HIP(ctx, hipMemcpyAsync(state, h_state0, sizeof h_state0, hipMemcpyHostToDevice, st));
+    uint32_t iter = n, n_total = n;
+    while (iter > 1) {
+        const uint32_t bound = iter;
+        for (uint32_t k = 0; k < kPlocBatch; ++k) {
+            hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(bound)), dim3(kThreads), 0, st, b_in, state,
+                               radius, nn);
+            RTQ_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ploc_flags, dim3(blocks_for(bound)), dim3(kThreads), 0, st, nn, state, bound, flags);
+            RTQ_HIP(ctx, hipGetLastError());
+            const rtq_status rs = scan_u32x2_ws(ctx, flags, offs, bound, tot, scan_ws, st);
+            if (rs != RTQ_OK) return rs;
+            hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(bound)), dim3(kThreads), 0, st, state, nn, flags, offs,
+                               c_in, b_in, c_out, b_out, s->cbox, s->cleft, s->cright, s->cparent, s->ccount,
+                               s->ctri);
+            RTQ_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_ploc_advance, dim3(1), dim3(1), 0, st, tot, state, pass_log, k);
+            RTQ_HIP(ctx, hipGetLastError());
+            std::swap(c_in, c_out);
+            std::swap(b_in, b_out);
+        }
+        uint4 h_log[kPlocBatch];
+        uint32_t h_state[2];
+        RTQ_HIP(ctx, hipMemcpyAsync(h_log, pass_log, sizeof h_log, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipMemcpyAsync(h_state, state, sizeof h_state, hipMemcpyDeviceToHost, st));
+        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        for (uint32_t k = 0; k < kPlocBatch; ++k) {
+            const uint4 e = h_log[k];  // (first merge id, merges, clusters at the pass)
+            if (e.z <= 1) break;
+            if (e.y == 0) return fail(ctx, RTQ_EINTERNAL, "PLOC pass with %u clusters made no merge", e.z);
+            pass_ranges.emplace_back(e.x, e.y);
+        }
+        iter = h_state[0];
+        n_total = h_state[1];
     }
-};
+    if (n_total != 2 * n - 1)
+        return fail(ctx, RTQ_EINTERNAL, "PLOC produced %u clusters, expected %u", n_total, 2 * n - 1);
+    return RTQ_OK;
+}
 
 // Steps after PLOC, shared by the build and the refit: pre-order placement,
 // the reference node array and leaf-ordered triangles, the box pad from the
@@ -578,7 +1095,7 @@ struct BuildTemps {
 rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipStream_t st) {
     const uint32_t n = s->n_tris;
     const uint32_t nc = s->n_clusters;
-    BuildTemps tmp;
+    Temps tmp(ctx);
     uint32_t *pos = nullptr, *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr,
              *bidx = nullptr, *ntrav = nullptr;
     RTQ_HIP(ctx, tmp.get(&pos, nc));
@@ -654,9 +1171,13 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     hipStream_t st = ctx->stream;
     const uint32_t n = s->n_tris;
     const uint32_t nc = 2 * n - 1;
-    free_build(s);
+    // work queued earlier may still read the buffers a rebuild overwrites
+    RTQ_HIP(ctx, hipDeviceSynchronize());
+    reset_build(s);
     s->params = prm;
     s->n_clusters = nc;
+    rtq_status rs = arena_reserve(ctx);
+    if (rs != RTQ_OK) return rs;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     RTQ_HIP(ctx, hipEventCreate(&ev0));
@@ -667,38 +1188,32 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     } evg{ev0, ev1};
     RTQ_HIP(ctx, hipEventRecord(ev0, st));
 
-    // persistent products
-    RTQ_HIP(ctx, dev_alloc(&s->cbox, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->ctri, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->cleft, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->cright, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->cparent, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->ccount, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->sorted_idx, n));
-    RTQ_HIP(ctx, dev_alloc(&s->sorted_codes, n));
-    RTQ_HIP(ctx, dev_alloc(&s->ref_nodes, nc));
-    RTQ_HIP(ctx, dev_alloc(&s->tris, 3ull * n));
+    // persistent products, sized by the triangle count: a rebuild rewrites them
+    if (!s->cbox) RTQ_HIP(ctx, dev_alloc(&s->cbox, nc));
+    if (!s->ctri) RTQ_HIP(ctx, dev_alloc(&s->ctri, nc));
+    if (!s->cleft) RTQ_HIP(ctx, dev_alloc(&s->cleft, nc));
+    if (!s->cright) RTQ_HIP(ctx, dev_alloc(&s->cright, nc));
+    if (!s->cparent) RTQ_HIP(ctx, dev_alloc(&s->cparent, nc));
+    if (!s->ccount) RTQ_HIP(ctx, dev_alloc(&s->ccount, nc));
+    if (!s->sorted_idx) RTQ_HIP(ctx, dev_alloc(&s->sorted_idx, n));
+    if (!s->sorted_codes) RTQ_HIP(ctx, dev_alloc(&s->sorted_codes, n));
+    if (!s->ref_nodes) RTQ_HIP(ctx, dev_alloc(&s->ref_nodes, nc));
+    if (!s->tris) RTQ_HIP(ctx, dev_alloc(&s->tris, 3ull * n));
 
-    BuildTemps tmp;
+    Temps tmp(ctx);
     rtq_aabb* d_bounds = nullptr;
-    uint32_t *tk = nullptr, *tv = nullptr, *c_in = nullptr, *c_out = nullptr, *nn = nullptr;
+    uint32_t *tk = nullptr, *tv = nullptr, *c_in = nullptr, *c_out = nullptr;
     Box6 *b_in = nullptr, *b_out = nullptr;
-    uint2 *flags = nullptr, *offs = nullptr, *tot = nullptr;
     RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));
     RTQ_HIP(ctx, tmp.get(&tk, n));
     RTQ_HIP(ctx, tmp.get(&tv, n));
     RTQ_HIP(ctx, tmp.get(&c_in, n));
     RTQ_HIP(ctx, tmp.get(&c_out, n));
-    RTQ_HIP(ctx, tmp.get(&nn, n));
     RTQ_HIP(ctx, tmp.get(&b_in, n));
     RTQ_HIP(ctx, tmp.get(&b_out, n));
-    RTQ_HIP(ctx, tmp.get(&flags, n));
-    RTQ_HIP(ctx, tmp.get(&offs, n));
-    RTQ_HIP(ctx, tmp.get(&tot, 1));
-    RTQ_HIP(ctx, hipMemsetAsync(nn, 0, sizeof(uint32_t) * n, st));
 
     // 1. bounds, cube, Morton codes, sort
-    rtq_status rs = compute_bounds(ctx, s, prm.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
+    rs = compute_bounds(ctx, s, prm.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
     if (rs != RTQ_OK) return rs;
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, n,
                        d_bounds, s->sorted_codes, s->sorted_idx);
@@ -713,54 +1228,11 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     hipLaunchKernelGGL(k_leaf_init, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models,
                        s->sorted_idx, n, s->cbox, s->ctri, s->ccount, c_in, b_in);
     RTQ_HIP(ctx, hipGetLastError());
-    // passes are queued kPlocBatch at a time against a device-side cluster
-    // count; the host reads the count and the pass log once per batch (passes
-    // queued after the last merge find one cluster and change nothing)
-    constexpr uint32_t kPlocBatch = 8;
     std::vector<std::pair<uint32_t, uint32_t>> pass_ranges;
-    uint32_t *state = nullptr;
-    uint4* pass_log = nullptr;
-    uint2* scan_ws = nullptr;
-    RTQ_HIP(ctx, tmp.get(&state, 2));
-    RTQ_HIP(ctx, tmp.get(&pass_log, kPlocBatch));
-    RTQ_HIP(ctx, tmp.get(&scan_ws, std::max<uint64_t>(scan_u32x2_workspace(n), 1)));
-    const uint32_t h_state0[2] = {n, n};
-    RTQ_HIP(ctx, hipMemcpyAsync(state, h_state0, sizeof h_state0, hipMemcpyHostToDevice, st));
-    uint32_t iter = n, n_total = n;
-    while (iter > 1) {
-        const uint32_t bound = iter;
-        for (uint32_t k = 0; k < kPlocBatch; ++k) {
-            hipLaunchKernelGGL(k_ploc_nn, dim3(blocks_for(bound)), dim3(kThreads), 0, st, b_in, state,
-                               prm.search_radius, nn);
-            RTQ_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_ploc_flags, dim3(blocks_for(bound)), dim3(kThreads), 0, st, nn, state, bound, flags);
-            RTQ_HIP(ctx, hipGetLastError());
-            rs = scan_u32x2_ws(ctx, flags, offs, bound, tot, scan_ws, st);
-            if (rs != RTQ_OK) return rs;
-            hipLaunchKernelGGL(k_ploc_merge, dim3(blocks_for(bound)), dim3(kThreads), 0, st, state, nn, flags, offs,
-                               c_in, b_in, c_out, b_out, s->cbox, s->cleft, s->cright, s->cparent, s->ccount,
-                               s->ctri);
-            RTQ_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_ploc_advance, dim3(1), dim3(1), 0, st, tot, state, pass_log, k);
-            RTQ_HIP(ctx, hipGetLastError());
-            std::swap(c_in, c_out);
-            std::swap(b_in, b_out);
-        }
-        uint4 h_log[kPlocBatch];
-        uint32_t h_state[2];
-        RTQ_HIP(ctx, hipMemcpyAsync(h_log, pass_log, sizeof h_log, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipMemcpyAsync(h_state, state, sizeof h_state, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipStreamSynchronize(st));
-        for (uint32_t k = 0; k < kPlocBatch; ++k) {
-            const uint4 e = h_log[k];  // (first merge id, merges, clusters at the pass)
-            if (e.z <= 1) break;
-            if (e.y == 0) return fail(ctx, RTQ_EINTERNAL, "PLOC pass with %u clusters made no merge", e.z);
-            pass_ranges.emplace_back(e.x, e.y);
-        }
-        iter = h_state[0];
-        n_total = h_state[1];
-    }
-    if (n_total != nc) return fail(ctx, RTQ_EINTERNAL, "PLOC produced %u clusters, expected %u", n_total, nc);
+    rs = (prm.flags & RTQ_BUILD_PLOC_SPLIT)
+             ? ploc_split(ctx, s, prm.search_radius, tmp, c_in, c_out, b_in, b_out, pass_ranges, st)
+             : ploc_fused(ctx, s, prm.search_radius, tmp, c_in, c_out, b_in, b_out, pass_ranges, st);
+    if (rs != RTQ_OK) return rs;
     s->ploc_passes = (uint32_t)pass_ranges.size();
     s->passes = pass_ranges;
 
@@ -773,6 +1245,7 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     RTQ_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
     s->build_ms = ms;
     s->built = true;
+    s->generation = ++ctx->builds;
     return RTQ_OK;
 }
 
@@ -801,7 +1274,9 @@ rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st) {
 rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
     hipStream_t st = ctx->stream;
     const uint32_t n = s->n_tris;
-    BuildTemps tmp;
+    const rtq_status ra = arena_reserve(ctx);
+    if (ra != RTQ_OK) return ra;
+    Temps tmp(ctx);
     rtq_aabb* d_bounds = nullptr;
     RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));
     hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, s->ctri,
@@ -819,7 +1294,10 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
 
 rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_codes) {
     hipStream_t st = ctx->stream;
-    BuildTemps tmp;
+    RTQ_HIP(ctx, hipDeviceSynchronize());  // nothing queued still uses the arena
+    const rtq_status ra = arena_reserve(ctx);
+    if (ra != RTQ_OK) return ra;
+    Temps tmp(ctx);
     rtq_aabb* d_bounds = nullptr;
     uint32_t* codes = nullptr;
     RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));

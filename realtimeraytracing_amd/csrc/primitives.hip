@@ -114,24 +114,18 @@ rtq_status scan_impl(rtq_ctx* ctx, const T* in, T* out, uint64_t n, T* total, hi
         RTQ_HIP(ctx, hipGetLastError());
         return RTQ_OK;
     }
+    Temps tmp(ctx);
     T* sums = nullptr;
-    RTQ_HIP(ctx, dev_alloc(&sums, 2 * nb));
+    RTQ_HIP(ctx, tmp.get(&sums, 2 * nb));
     T* offs = sums + nb;
     hipLaunchKernelGGL(k_tile_reduce<T>, dim3((uint32_t)nb), dim3(kScanThreads), 0, st, in, n, sums);
-    hipError_t le = hipGetLastError();
-    rtq_status s = (le == hipSuccess) ? scan_impl<T>(ctx, sums, offs, nb, nullptr, st)
-                                      : hip_fail(ctx, le, "k_tile_reduce");
-    if (s == RTQ_OK) {
-        hipLaunchKernelGGL(k_tile_scan<T>, dim3((uint32_t)nb), dim3(kScanThreads), 0, st, in, out, n,
-                           (const T*)offs, total);
-        le = hipGetLastError();
-        if (le != hipSuccess) s = hip_fail(ctx, le, "k_tile_scan");
-    }
-    // the temporary is released once the queued kernels are done
-    hipError_t se = hipStreamSynchronize(st);
-    (void)hipFree(sums);
-    if (s == RTQ_OK && se != hipSuccess) s = hip_fail(ctx, se, "hipStreamSynchronize");
-    return s;
+    RTQ_HIP(ctx, hipGetLastError());
+    const rtq_status s = scan_impl<T>(ctx, sums, offs, nb, nullptr, st);
+    if (s != RTQ_OK) return s;
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3((uint32_t)nb), dim3(kScanThreads), 0, st, in, out, n,
+                       (const T*)offs, total);
+    RTQ_HIP(ctx, hipGetLastError());
+    return RTQ_OK;
 }
 
 // Elements of workspace scan_impl_ws needs for n elements (the tile sums and
@@ -269,11 +263,12 @@ rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32
     const uint32_t nn = (uint32_t)n;
     const uint32_t n_tiles = (nn + kSortTile - 1) / kSortTile;
     // digit counts, their offsets and the offsets scan's workspace in one
-    // allocation, so the passes queue without synchronising
+    // block, so the passes queue without synchronising
     const uint64_t n_hist = 256ull * n_tiles;
     const uint64_t ws = scan_workspace<U1>(n_hist);
+    Temps tmp(ctx);
     uint32_t* hist = nullptr;
-    RTQ_HIP(ctx, dev_alloc(&hist, 2ull * n_hist + ws));
+    RTQ_HIP(ctx, tmp.get(&hist, 2ull * n_hist + ws));
     uint32_t* offs = hist + n_hist;
     U1* scan_ws = reinterpret_cast<U1*>(offs + n_hist);
     uint32_t *src_k = keys, *src_v = vals, *dst_k = tmp_k, *dst_v = tmp_v;
@@ -300,9 +295,6 @@ rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32
             e = hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) s = hip_fail(ctx, e, "sort copy-back");
     }
-    hipError_t se = hipStreamSynchronize(st);
-    (void)hipFree(hist);
-    if (s == RTQ_OK && se != hipSuccess) s = hip_fail(ctx, se, "hipStreamSynchronize");
     return s;
 }
 

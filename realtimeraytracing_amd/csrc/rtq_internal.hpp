@@ -3,10 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/rtq/rtq.h"
@@ -83,6 +85,9 @@ struct Scene {
     bool stale = false;
     float box_pad = 0.0f;
     float build_ms = 0.0f;
+    // set by each build from the context's counter, 0 while unbuilt: a batch
+    // graph records it and refuses to replay over buffers of another build
+    uint64_t generation = 0;
 };
 
 }  // namespace rtq
@@ -99,6 +104,18 @@ struct TraceScratch {
 };
 }  // namespace rtq
 
+namespace rtq {
+// Device scratch kept by a context for the temporaries of builds, refits,
+// scans and sorts (see Temps).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t top = 0;    // bytes handed out
+    size_t extra = 0;  // bytes of live allocations made because the arena was full
+    size_t peak = 0;   // largest top + extra seen
+};
+}  // namespace rtq
+
 struct rtq_ctx {
     int device = 0;
     int n_cu = 0;
@@ -106,6 +123,8 @@ struct rtq_ctx {
     std::string err;
     std::map<uint32_t, rtq::Scene*> scenes;
     uint32_t next_scene = 1;
+    uint64_t builds = 0;  // build generations handed out
+    rtq::Arena arena;
     rtq::TraceScratch scratch;
     // traversal launch settings: refill a wave once `refill` lanes are idle,
     // taking `chunk` consecutive rays per atomic on a work counter; trace
@@ -124,6 +143,7 @@ struct rtq_ctx {
 struct rtq_batch {
     rtq_ctx* ctx = nullptr;
     std::vector<rtq_batch_item> items;
+    std::vector<uint64_t> generations;  // of each item's scene at capture
     rtq::TraceScratch scratch;
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
@@ -137,14 +157,67 @@ rtq_status hip_fail(rtq_ctx* ctx, hipError_t e, const char* what);
 Scene* find_scene(rtq_ctx* ctx, uint32_t id);
 hipStream_t pick_stream(rtq_ctx* ctx, void* stream);
 void free_build(Scene* s);
+void reset_build(Scene* s);
 
 template <typename T>
 hipError_t dev_alloc(T** p, size_t count) {
     return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
 }
 
+// Temporaries of one scope, taken from the top of the context's arena and
+// given back when the Temps goes out of scope (scopes nest).  Every user
+// queues its work on one stream or waits for it before returning, so memory
+// given back is taken again only by kernels queued after the ones that used
+// it.  A request past the arena's capacity gets its own allocation (freed,
+// with an implicit device synchronisation, by the Temps) and raises the
+// arena's peak; arena_reserve grows the arena to that peak between
+// operations, so repeating an operation allocates nothing.
+class Temps {
+  public:
+    explicit Temps(rtq_ctx* ctx) : a_(&ctx->arena), mark_(ctx->arena.top) {}
+    Temps(const Temps&) = delete;
+    Temps& operator=(const Temps&) = delete;
+    ~Temps() {
+        for (const auto& o : own_) {
+            (void)hipFree(o.first);
+            a_->extra -= o.second;
+        }
+        a_->top = mark_;
+    }
+    template <typename T>
+    hipError_t get(T** p, size_t count) {
+        const size_t bytes = (sizeof(T) * (count ? count : 1) + 255) & ~size_t(255);
+        if (a_->top + bytes <= a_->cap) {
+            *p = reinterpret_cast<T*>(a_->base + a_->top);
+            a_->top += bytes;
+        } else {
+            void* q = nullptr;
+            const hipError_t e = hipMalloc(&q, bytes);
+            if (e != hipSuccess) {
+                *p = nullptr;
+                return e;
+            }
+            own_.emplace_back(q, bytes);
+            a_->extra += bytes;
+            *p = static_cast<T*>(q);
+        }
+        a_->peak = std::max(a_->peak, a_->top + a_->extra);
+        return hipSuccess;
+    }
+
+  private:
+    Arena* a_;
+    size_t mark_;
+    std::vector<std::pair<void*, size_t>> own_;
+};
+
+// At the start of an operation (no Temps alive): grow the arena to the
+// largest total an earlier operation needed.
+rtq_status arena_reserve(rtq_ctx* ctx);
+
 // Exclusive scan of u32 on device (wrapping); `total` (device, may be null)
-// receives the sum.  Temporary storage is allocated internally.
+// receives the sum.  Temporaries come from the context's arena; the scan is
+// queued on st and not waited for.
 rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
                     uint32_t* total, hipStream_t st);
 // Exclusive scan of pairs of u32 (component-wise).
@@ -170,6 +243,7 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s);
 // words change (build.hip).
 rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
+// Queued on st and not waited for; digit counts come from the context's arena.
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
                             uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st);
 

@@ -669,16 +669,21 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(W
 // the 2M-triangle terrain: the batched binary kernel at 7 waves per SIMD
 // instead of 6 (74 -> 72 VGPRs, no spill), the single-scene 8-wide kernels at
 // 7 instead of 6 (74 -> 72 VGPRs, no spill); the single-scene binary kernels
-// were slower when pushed from 7 to 8 waves.
+// were slower when pushed from 7 to 8 waves, and the batched 8-wide kernel
+// (90 VGPRs, 5 waves) spills at 6 or 7 waves for no gain.
 #ifndef RTQ_OCC_BATCH
 #define RTQ_OCC_BATCH 7
 #endif
 #ifndef RTQ_OCC_WIDE
 #define RTQ_OCC_WIDE 7
 #endif
+#ifndef RTQ_OCC_BATCH_WIDE
+#define RTQ_OCC_BATCH_WIDE 0
+#endif
 template <int LAYOUT, bool MULTI>
 constexpr int occupancy_hint() {
-    return MULTI ? (LAYOUT == kBinary ? RTQ_OCC_BATCH : 0) : (LAYOUT == kWide ? RTQ_OCC_WIDE : 0);
+    return MULTI ? (LAYOUT == kBinary ? RTQ_OCC_BATCH : (LAYOUT == kWide ? RTQ_OCC_BATCH_WIDE : 0))
+                 : (LAYOUT == kWide ? RTQ_OCC_WIDE : 0);
 }
 
 template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>

@@ -173,23 +173,14 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st
     const uint32_t cap = std::max<uint32_t>(s->n_trav, 1);  // wide nodes are a subset of the binary ones
     uint32_t *items = nullptr, *next = nullptr, *children = nullptr, *nchild = nullptr, *nexp = nullptr,
              *offs = nullptr, *total = nullptr;
-    std::vector<void*> temps;
-    auto get = [&](uint32_t** p, size_t count) {
-        hipError_t e = dev_alloc(p, count);
-        if (e == hipSuccess) temps.push_back(*p);
-        return e;
-    };
-    struct Free {
-        std::vector<void*>& t;
-        ~Free() { for (void* p : t) (void)hipFree(p); }
-    } guard{temps};
-    RTQ_HIP(ctx, get(&items, cap));
-    RTQ_HIP(ctx, get(&next, cap));
-    RTQ_HIP(ctx, get(&children, (size_t)cap * kWidth));
-    RTQ_HIP(ctx, get(&nchild, cap));
-    RTQ_HIP(ctx, get(&nexp, cap));
-    RTQ_HIP(ctx, get(&offs, cap));
-    RTQ_HIP(ctx, get(&total, 1));
+    Temps tmp(ctx);
+    RTQ_HIP(ctx, tmp.get(&items, cap));
+    RTQ_HIP(ctx, tmp.get(&next, cap));
+    RTQ_HIP(ctx, tmp.get(&children, (size_t)cap * kWidth));
+    RTQ_HIP(ctx, tmp.get(&nchild, cap));
+    RTQ_HIP(ctx, tmp.get(&nexp, cap));
+    RTQ_HIP(ctx, tmp.get(&offs, cap));
+    RTQ_HIP(ctx, tmp.get(&total, 1));
     // a refit writes the same buffer again (its size depends on the topology only)
     if (!s->wide) RTQ_HIP(ctx, dev_alloc(&s->wide, 8ull * cap));
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, items, root);

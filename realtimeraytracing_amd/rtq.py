@@ -34,6 +34,7 @@ HIT_DTYPE = np.dtype({"names": ["id", "t", "u", "v"], "formats": ["<i4", "<f4", 
 
 RTQ_OK, RTQ_EINVAL, RTQ_ENOMEM, RTQ_EHIP, RTQ_ECAPACITY, RTQ_EINTERNAL, RTQ_ENOTBUILT = range(7)
 RTQ_BUILD_REF_PADDED_BOUNDS = 0x1
+RTQ_BUILD_PLOC_SPLIT = 0x2
 RTQ_TRACE_CLOSEST = 0x0
 RTQ_TRACE_ANY = 0x1
 RTQ_TRACE_LEAF_REFMT = 0x2

@@ -169,3 +169,40 @@ def test_update_errors(rtq_ctx):
         sc.trace_host(rays_for(16))                      # rejected updates leave the scene as it was
     finally:
         sc.release()
+
+
+@pytest.mark.gpu
+def test_batch_refuses_stale_or_rebuilt_scene(rtq_ctx, torch_mod):
+    """A batch graph holds its scenes' node buffers: it replays after a refit
+    (buffers kept), and refuses after an edit without refit or a rebuild."""
+    tris = scenes.soup(3000, seed=12)
+    models = scenes.identity_models()
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        sc.build(16, 4, 0)
+        rays = rays_for(2048, seed=2)
+        rd = devmem.to_device(rays)
+        hd = devmem.empty(len(rays), HIT_DTYPE)
+        torch_mod.cuda.synchronize()
+        graph = rtq_ctx.batch(rtq_ctx.items([(sc, rd.data_ptr(), len(rays), hd.data_ptr(), 0)]))
+        try:
+            graph.launch()
+            new = moved(tris, amount=0.01)
+            sc.update(new)
+            with pytest.raises(RtqError) as e:
+                graph.launch()
+            assert e.value.status == RTQ_ENOTBUILT
+            sc.refit()
+            graph.launch()
+            rtq_ctx.synchronize()
+            ref = oracle.trace_brute(oracle.world_vertices(new, models), rays)
+            got = devmem.to_host(hd, HIT_DTYPE)
+            assert hits_equal(got, ref), first_diff(got, ref)
+            sc.build(16, 4, 0)
+            with pytest.raises(RtqError) as e:
+                graph.launch()
+            assert e.value.status == RTQ_ENOTBUILT
+        finally:
+            graph.destroy()
+    finally:
+        sc.release()

@@ -71,13 +71,8 @@ __global__ __launch_bounds__(kThreads) void k_refit_leaves(const rtq_triangle* _
 
 // Merged clusters [first, first + count) of one PLOC pass: the box is the
 // merge of the children's (left operand first, as k_ploc_merge).
-__global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32_t count,
-                                                          const uint32_t* __restrict__ cleft,
-                                                          const uint32_t* __restrict__ cright,
-                                                          rtq_aabb* __restrict__ cbox) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= count) return;
-    const uint32_t id = first + i;
+__device__ __forceinline__ void refit_merge_one(uint32_t id, const uint32_t* __restrict__ cleft,
+                                                const uint32_t* __restrict__ cright, rtq_aabb* __restrict__ cbox) {
     const rtq_aabb l = cbox[cleft[id]], r = cbox[cright[id]];
     rtq_aabb a{};
     for (int k = 0; k < 3; ++k) {
@@ -85,6 +80,32 @@ __global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32
         a.max[k] = smax(l.max[k], r.max[k]);
     }
     cbox[id] = a;
+}
+
+__global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32_t count,
+                                                          const uint32_t* __restrict__ cleft,
+                                                          const uint32_t* __restrict__ cright,
+                                                          rtq_aabb* __restrict__ cbox) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < count) refit_merge_one(first + i, cleft, cright, cbox);
+}
+
+// Passes that merge few clusters (the top of the tree) would each be a tiny
+// launch: one workgroup runs passes [k0, k1) of `passes` (first id, merges)
+// in order, a barrier between passes (a pass reads boxes the previous ones
+// wrote, which the workgroup's own stores and barrier make visible).
+constexpr uint32_t kSmallPass = 4096;
+constexpr int kSmallThreads = 1024;
+
+__global__ __launch_bounds__(kSmallThreads) void k_refit_merge_small(const uint2* __restrict__ passes, uint32_t k0,
+                                                                     uint32_t k1, const uint32_t* __restrict__ cleft,
+                                                                     const uint32_t* __restrict__ cright,
+                                                                     rtq_aabb* __restrict__ cbox) {
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint2 p = passes[k];
+        for (uint32_t i = threadIdx.x; i < p.y; i += kSmallThreads) refit_merge_one(p.x + i, cleft, cright, cbox);
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------- top of the tree
@@ -798,6 +819,21 @@ __global__ void k_root_init(uint32_t root, uint32_t* pos, uint32_t* fl, uint32_t
     dep[root] = 0;
 }
 
+__device__ __forceinline__ void place_one(uint32_t p, const uint32_t* __restrict__ cleft,
+                                          const uint32_t* __restrict__ cright, const uint32_t* __restrict__ ccount,
+                                          uint32_t* __restrict__ pos, uint32_t* __restrict__ fl,
+                                          uint32_t* __restrict__ dep) {
+    const uint32_t l = cleft[p], r = cright[p];
+    const uint32_t pp = pos[p], pf = fl[p], pd = dep[p];
+    const uint32_t lc = ccount[l];
+    pos[l] = pp + 1;
+    pos[r] = pp + 2 * lc;
+    fl[l] = pf;
+    fl[r] = pf + lc;
+    dep[l] = pd + 1;
+    dep[r] = pd + 1;
+}
+
 // Top-down pre-order placement for the clusters created in one PLOC pass
 // (their parents were created in later passes, so are already placed):
 // left child right after the parent, right child after the left subtree
@@ -810,17 +846,23 @@ __global__ __launch_bounds__(kThreads) void k_place_pass(uint32_t first, uint32_
                                                          uint32_t* __restrict__ fl,
                                                          uint32_t* __restrict__ dep) {
     const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= count) return;
-    const uint32_t p = first + k;
-    const uint32_t l = cleft[p], r = cright[p];
-    const uint32_t pp = pos[p], pf = fl[p], pd = dep[p];
-    const uint32_t lc = ccount[l];
-    pos[l] = pp + 1;
-    pos[r] = pp + 2 * lc;
-    fl[l] = pf;
-    fl[r] = pf + lc;
-    dep[l] = pd + 1;
-    dep[r] = pd + 1;
+    if (k < count) place_one(first + k, cleft, cright, ccount, pos, fl, dep);
+}
+
+// The same for passes [k0, k1), last pass first (the top of the tree), in one
+// workgroup with a barrier between passes.
+__global__ __launch_bounds__(kSmallThreads) void k_place_small(const uint2* __restrict__ passes, uint32_t k0,
+                                                               uint32_t k1, const uint32_t* __restrict__ cleft,
+                                                               const uint32_t* __restrict__ cright,
+                                                               const uint32_t* __restrict__ ccount,
+                                                               uint32_t* __restrict__ pos, uint32_t* __restrict__ fl,
+                                                               uint32_t* __restrict__ dep) {
+    for (uint32_t k = k1; k > k0; --k) {
+        const uint2 p = passes[k - 1];
+        for (uint32_t i = threadIdx.x; i < p.y; i += kSmallThreads)
+            place_one(p.x + i, cleft, cright, ccount, pos, fl, dep);
+        __syncthreads();
+    }
 }
 
 // Reference node array (scene.cpp:189-208) + leaf-ordered traversal triangles.
@@ -1109,9 +1151,18 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     const uint32_t root = nc - 1;
     hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
     RTQ_HIP(ctx, hipGetLastError());
-    for (auto it = s->passes.rbegin(); it != s->passes.rend(); ++it) {
-        hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(it->second)), dim3(kThreads), 0, st, it->first,
-                           it->second, s->cleft, s->cright, s->ccount, pos, fl, dep);
+    // the small passes at the top of the tree in one launch, then one launch
+    // per larger pass, last pass first
+    const uint32_t n_passes = (uint32_t)s->passes.size();
+    if (s->small_from < n_passes) {
+        hipLaunchKernelGGL(k_place_small, dim3(1), dim3(kSmallThreads), 0, st, s->d_passes, s->small_from, n_passes,
+                           s->cleft, s->cright, s->ccount, pos, fl, dep);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    for (uint32_t k = s->small_from; k > 0; --k) {
+        const auto& pr = s->passes[k - 1];
+        hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
+                           s->cleft, s->cright, s->ccount, pos, fl, dep);
         RTQ_HIP(ctx, hipGetLastError());
     }
     hipLaunchKernelGGL(k_write_ref, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->cbox, s->ctri,
@@ -1235,6 +1286,19 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     if (rs != RTQ_OK) return rs;
     s->ploc_passes = (uint32_t)pass_ranges.size();
     s->passes = pass_ranges;
+    // device copy of the pass ranges for the single-launch small passes
+    // (placement here, box merges in a refit): the passes from small_from on
+    // merge at most kSmallPass clusters each
+    s->small_from = s->ploc_passes;
+    while (s->small_from > 0 && s->passes[s->small_from - 1].second <= kSmallPass) --s->small_from;
+    if (s->ploc_passes > 0) {
+        s->h_passes.resize(s->ploc_passes);
+        for (uint32_t k = 0; k < s->ploc_passes; ++k)
+            s->h_passes[k] = make_uint2(s->passes[k].first, s->passes[k].second);
+        RTQ_HIP(ctx, dev_alloc(&s->d_passes, s->ploc_passes));
+        RTQ_HIP(ctx, hipMemcpyAsync(s->d_passes, s->h_passes.data(), sizeof(uint2) * s->ploc_passes,
+                                    hipMemcpyHostToDevice, st));
+    }
 
     // 3-4. node arrays of every layout
     rs = write_layouts(ctx, s, d_bounds, st);
@@ -1282,9 +1346,16 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
     hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, s->ctri,
                        n, s->cbox);
     RTQ_HIP(ctx, hipGetLastError());
-    for (const auto& pr : s->passes) {
+    const uint32_t n_passes = (uint32_t)s->passes.size();
+    for (uint32_t k = 0; k < s->small_from; ++k) {
+        const auto& pr = s->passes[k];
         hipLaunchKernelGGL(k_refit_merge, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
                            s->cleft, s->cright, s->cbox);
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    if (s->small_from < n_passes) {
+        hipLaunchKernelGGL(k_refit_merge_small, dim3(1), dim3(kSmallThreads), 0, st, s->d_passes, s->small_from,
+                           n_passes, s->cleft, s->cright, s->cbox);
         RTQ_HIP(ctx, hipGetLastError());
     }
     rtq_status rs = compute_bounds(ctx, s, s->params.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);

@@ -81,6 +81,11 @@ struct Scene {
     uint32_t ploc_passes = 0;
     // (first cluster id, count) of the merges of each PLOC pass, in build order
     std::vector<std::pair<uint32_t, uint32_t>> passes;
+    // the same on the device (and its host staging copy); passes from
+    // small_from on are small enough to run in one workgroup
+    uint2* d_passes = nullptr;
+    std::vector<uint2> h_passes;
+    uint32_t small_from = 0;
     // vertices changed since the last build or refit (rtq_scene_update)
     bool stale = false;
     float box_pad = 0.0f;

@@ -230,12 +230,16 @@ enum Layout : int {
     kOrdered = 2,  // 64-B two-child nodes, learned child order where both are hit
 };
 
-// MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: rays come from
-// several items (scenes), each lane takes its scene and buffers from its item.
+// MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: 0 one ray set;
+// 1 several items (ray sets over any scenes), each lane takes its scene from
+// its item; 2 several items over one scene, whose nodes and triangles are
+// kernel arguments (uniform registers instead of two per-lane pointers).  A
+// batched lane keeps its item index and looks up the hit buffer when it
+// writes the hit.
 // The body of the trace kernels below (k_trace, and k_trace_occ with an
 // occupancy hint).
-template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
-__device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<MULTI>& table) {
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
+__device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<(MULTI != 0)>& table) {
     constexpr uint32_t kStackLds = LAYOUT == kWide ? kStackLdsWide : kStackLdsBinary;
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
@@ -257,7 +261,8 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
     const float4* nodes_p = a.nodes;
     const float4* tris_p = a.tris;
     rtq_hit* hits_p = a.hits;
-    uint32_t local = 0;              // ray index within its ray set
+    uint32_t local = 0;              // ray index within its ray set (one ray set)
+    uint32_t lane_item = 0;          // batch item of the lane's ray
     bool lane_any = MODE == 1;
     // learned order: bit (octant + 8 * any) of a node's order word set = child 0
     // first, bit 16 + that = decision used; order_mask clears the word for
@@ -321,7 +326,12 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             h.u = 0.0f;
             h.v = 0.0f;
         }
-        hits_p[local] = h;
+        if constexpr (MULTI != 0) {
+            const BatchItem& it = s_items[lane_item];
+            it.hits[(uint32_t)rid - it.begin] = h;
+        } else {
+            hits_p[local] = h;
+        }
         if (STATS) {
             if (a.stat_nodes) a.stat_nodes[rid] = n_nodes;
             if (a.stat_tris) a.stat_tris[rid] = n_tests;
@@ -505,12 +515,14 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                     rid = (int)r;
                     const rtq_ray* rays_p = a.rays;
                     local = r;
-                    if constexpr (MULTI) {
+                    if constexpr (MULTI != 0) {
                         const BatchItem& it = s_items[pool_item];
-                        nodes_p = it.nodes;
-                        tris_p = it.tris;
+                        if constexpr (MULTI == 1) {
+                            nodes_p = it.nodes;
+                            tris_p = it.tris;
+                        }
                         rays_p = it.rays;
-                        hits_p = it.hits;
+                        lane_item = pool_item;
                         local = r - it.begin;
                         lane_any = it.any != 0;
                         order_mask = it.learned ? 0xFFFFFFFFu : 0u;
@@ -652,25 +664,27 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
     }
 }
 
-template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
-__global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(TraceArgs a, ItemTable<MULTI> table) {
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
+__global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(TraceArgs a,
+                                                                            ItemTable<(MULTI != 0)> table) {
     trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI>(a, table);
 }
 
 // The same kernel with an occupancy (waves per SIMD) the register allocation
 // must fit.
-template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI, int WAVES>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int WAVES>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace_occ(
-    TraceArgs a, ItemTable<MULTI> table) {
+    TraceArgs a, ItemTable<(MULTI != 0)> table) {
     trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI>(a, table);
 }
 
 // Occupancy hints by layout (0 = the compiler's own choice), from timings on
-// the 2M-triangle terrain: the batched binary kernel at 7 waves per SIMD
-// instead of 6 (74 -> 72 VGPRs, no spill), the single-scene 8-wide kernels at
-// 7 instead of 6 (74 -> 72 VGPRs, no spill); the single-scene binary kernels
-// were slower when pushed from 7 to 8 waves, and the batched 8-wide kernel
-// (90 VGPRs, 5 waves) spills at 6 or 7 waves for no gain.
+// the 2M-triangle terrain: the batched binary kernels at least 7 waves per
+// SIMD (no spill), the single-scene 8-wide kernels at 7 instead of 6 (74 -> 72
+// VGPRs, no spill), the one-scene batched 8-wide kernel at 6 instead of 5
+// (80 VGPRs, no spill: +10% on the batched form); the single-scene binary
+// kernels were slower when pushed from 7 to 8 waves, and the multi-scene
+// batched 8-wide kernel (89 VGPRs, per-lane scene pointers) spills at 6.
 #ifndef RTQ_OCC_BATCH
 #define RTQ_OCC_BATCH 7
 #endif
@@ -680,13 +694,17 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(W
 #ifndef RTQ_OCC_BATCH_WIDE
 #define RTQ_OCC_BATCH_WIDE 0
 #endif
-template <int LAYOUT, bool MULTI>
+#ifndef RTQ_OCC_BATCH_WIDE_ONE
+#define RTQ_OCC_BATCH_WIDE_ONE 6
+#endif
+template <int LAYOUT, int MULTI>
 constexpr int occupancy_hint() {
-    return MULTI ? (LAYOUT == kBinary ? RTQ_OCC_BATCH : (LAYOUT == kWide ? RTQ_OCC_BATCH_WIDE : 0))
-                 : (LAYOUT == kWide ? RTQ_OCC_WIDE : 0);
+    if constexpr (MULTI == 0) return LAYOUT == kWide ? RTQ_OCC_WIDE : 0;
+    if constexpr (LAYOUT == kWide) return MULTI == 2 ? RTQ_OCC_BATCH_WIDE_ONE : RTQ_OCC_BATCH_WIDE;
+    return RTQ_OCC_BATCH;  // binary, with or without the learned order
 }
 
-template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
 auto trace_kernel() {
     constexpr int w = occupancy_hint<LAYOUT, MULTI>();
     if constexpr (w > 0)
@@ -697,9 +715,9 @@ auto trace_kernel() {
 
 // Persistent grid sized from occupancy, spill column sized from the stack
 // bound `depth_entries`, one launch (or only the scratch sizing).
-template <int LEAF, int MODE, bool STATS, int LAYOUT, bool MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
 rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const TraceArgs& base,
-                  const ItemTable<MULTI>& table, uint32_t n, bool size_only, hipStream_t st) {
+                  const ItemTable<(MULTI != 0)>& table, uint32_t n, bool size_only, hipStream_t st) {
     const auto kern = trace_kernel<LEAF, MODE, STATS, LAYOUT, MULTI>();
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTraceThreads, 0);
@@ -757,9 +775,9 @@ rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, 
     a.n_top = wide ? 0u : s->n_top;
     TraceScratch& sc = ctx->scratch;
     const uint64_t depth = stack_bound(s, wide);
-    if (layout == kWide) return launch<LEAF, MODE, STATS, kWide, false>(ctx, sc, depth, a, none, n, false, st);
-    if (layout == kOrdered) return launch<LEAF, MODE, STATS, kOrdered, false>(ctx, sc, depth, a, none, n, false, st);
-    return launch<LEAF, MODE, STATS, kBinary, false>(ctx, sc, depth, a, none, n, false, st);
+    if (layout == kWide) return launch<LEAF, MODE, STATS, kWide, 0>(ctx, sc, depth, a, none, n, false, st);
+    if (layout == kOrdered) return launch<LEAF, MODE, STATS, kOrdered, 0>(ctx, sc, depth, a, none, n, false, st);
+    return launch<LEAF, MODE, STATS, kBinary, 0>(ctx, sc, depth, a, none, n, false, st);
 }
 
 // Layout traced for a scene and trace flags; kNone-like -1 with an error set
@@ -822,6 +840,21 @@ rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, 
                : launch_single<1, 0, true>(ctx, s, args, nn, layout, st);
 }
 
+template <int MULTI>
+rtq_status launch_batch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth, const TraceArgs& args,
+                        const ItemTable<true>& table, uint32_t nt, bool size_only, hipStream_t st, bool refmt,
+                        int layout) {
+    if (refmt) {
+        if (layout == kWide) return launch<1, 2, false, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+        if (layout == kOrdered)
+            return launch<1, 2, false, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+        return launch<1, 2, false, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    }
+    if (layout == kWide) return launch<0, 2, false, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (layout == kOrdered) return launch<0, 2, false, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    return launch<0, 2, false, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+}
+
 // Up to kMaxBatchItems ray sets (any scenes, same leaf test and layout) in one
 // persistent launch: waves take chunks of the concatenated ray index space
 // (their XCD's part first, then any other part), so uneven sets balance across
@@ -871,25 +904,24 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
     TraceArgs args{};
     args.n = (uint32_t)total;
     args.n_items = n_items;
-    // one top-of-tree copy per launch: when every item traces the same binary nodes
+    // one scene for every item: its nodes and triangles become kernel
+    // arguments (MULTI 2), and binary layouts stage its top-of-tree copy
     const Scene* s0 = find_scene(ctx, items[0].scene_id);
-    bool one_tree = layout != kWide;
-    for (uint32_t i = 1; i < n_items && one_tree; ++i) one_tree = table.item[i].nodes == table.item[0].nodes;
-    if (one_tree) {
-        args.top = s0->top;
-        args.n_top = s0->n_top;
+    bool one_scene = true;
+    for (uint32_t i = 1; i < n_items && one_scene; ++i)
+        one_scene = table.item[i].nodes == table.item[0].nodes && table.item[i].tris == table.item[0].tris;
+    if (one_scene) {
+        args.nodes = table.item[0].nodes;
+        args.tris = table.item[0].tris;
+        if (layout != kWide) {
+            args.top = s0->top;
+            args.n_top = s0->n_top;
+        }
     }
     TraceScratch& sc = scratch ? *scratch : ctx->scratch;
     const uint32_t nt = (uint32_t)total;
-    if (leaf_flags) {
-        if (layout == kWide) return launch<1, 2, false, kWide, true>(ctx, sc, depth, args, table, nt, size_only, st);
-        if (layout == kOrdered)
-            return launch<1, 2, false, kOrdered, true>(ctx, sc, depth, args, table, nt, size_only, st);
-        return launch<1, 2, false, kBinary, true>(ctx, sc, depth, args, table, nt, size_only, st);
-    }
-    if (layout == kWide) return launch<0, 2, false, kWide, true>(ctx, sc, depth, args, table, nt, size_only, st);
-    if (layout == kOrdered) return launch<0, 2, false, kOrdered, true>(ctx, sc, depth, args, table, nt, size_only, st);
-    return launch<0, 2, false, kBinary, true>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (one_scene) return launch_batch<2>(ctx, sc, depth, args, table, nt, size_only, st, leaf_flags != 0, layout);
+    return launch_batch<1>(ctx, sc, depth, args, table, nt, size_only, st, leaf_flags != 0, layout);
 }
 
 }  // namespace rtq

@@ -242,12 +242,6 @@ rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
                          reinterpret_cast<U1*>(total), st);
 }
 
-rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total,
-                      hipStream_t st) {
-    return scan_impl<U2>(ctx, reinterpret_cast<const U2*>(in), reinterpret_cast<U2*>(out), n,
-                         reinterpret_cast<U2*>(total), st);
-}
-
 uint64_t scan_u32x2_workspace(uint64_t n) { return scan_workspace<U2>(n); }
 
 rtq_status scan_u32x2_ws(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total, uint2* ws,

@@ -225,11 +225,8 @@ rtq_status arena_reserve(rtq_ctx* ctx);
 // queued on st and not waited for.
 rtq_status scan_u32(rtq_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n,
                     uint32_t* total, hipStream_t st);
-// Exclusive scan of pairs of u32 (component-wise).
-rtq_status scan_u32x2(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total,
-                      hipStream_t st);
-// The same with a caller workspace of scan_u32x2_workspace(n) elements: no
-// allocation, no synchronisation (n > 0).
+// Exclusive scan of pairs of u32 (component-wise) with a caller workspace of
+// scan_u32x2_workspace(n) elements: no allocation, no synchronisation (n > 0).
 uint64_t scan_u32x2_workspace(uint64_t n);
 rtq_status scan_u32x2_ws(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, uint2* total, uint2* ws,
                          hipStream_t st);

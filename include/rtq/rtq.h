@@ -123,7 +123,9 @@ void rtq_build_params_default(rtq_build_params* p);
 /* GPU build: scene bounds (bvh.cpp:235-251), circumscribed cube (253-303),
  * centroids and Morton codes (305-372), stable (code, index) sort (214-232),
  * PLOC (26-210) and the pre-order flatten (scene.cpp:189-208), then the
- * traversal layout.  Replaces BVH::BVH (bvh.cpp:11-24). */
+ * traversal layout.  Replaces BVH::BVH (bvh.cpp:11-24).  Waits for work queued
+ * on the device first: a rebuild rewrites the scene's arrays in place.
+ * Temporaries come from the context's scratch, which keeps its largest size. */
 rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params* params);
 
 /* Refit: the built tree's topology kept (PLOC merges, leaf order, node

@@ -100,6 +100,21 @@ def test_fused_and_split_ploc_agree(rtq_ctx, n, radius):
         sc.release()
 
 
+@pytest.mark.parametrize("kind", ["copies", "identical"])
+def test_fused_ploc_with_ties(rtq_ctx, kind):
+    """Equal Morton codes and equal merged areas across tile boundaries of the
+    fused pass: 2,500 distinct triangles 4 times over, or one triangle 5,000
+    times (every neighbour distance ties, so the lowest index wins and a pass
+    merges about one pair: 4,999 passes, most of them in the tail)."""
+    base = scenes.soup(2500, seed=11)
+    tris = np.concatenate([base] * 4) if kind == "copies" else np.concatenate([base[:1]] * 5000)
+    models = scenes.identity_models()
+    sc, _, _ = _check_build(rtq_ctx, tris, models)
+    sc.release()
+    sc, _, _ = _check_build(rtq_ctx, tris, models, flags=RTQ_BUILD_PLOC_SPLIT)
+    sc.release()
+
+
 def test_padded_bounds_flag(rtq_ctx):
     # a scene away from the origin: the reference's zero padding moves the cube
     tris = scenes.soup(500, seed=3, origin=(2.0, 3.0, 4.0))

@@ -70,6 +70,22 @@ __global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t n_items, co
             ch[best] = cleft[e];
             ch[n++] = cright[e];
         }
+        // slots left over once every child is leaf-sized: split the largest
+        // leaf-sized clusters into smaller leaves (a visit tests all 8 slots
+        // anyway, and smaller leaf boxes cull more triangle tests)
+        while (n < kWidth) {
+            int best = -1;
+            float best_a = -1.0f;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (cleft[ch[i]] == kNone) continue;
+                const float a = box_area(cbox[ch[i]]);
+                if (a > best_a) { best_a = a; best = (int)i; }
+            }
+            if (best < 0) break;
+            const uint32_t e = ch[best];
+            ch[best] = cleft[e];
+            ch[n++] = cright[e];
+        }
     }
     uint32_t ne = 0;
     for (uint32_t i = 0; i < kWidth; ++i) {

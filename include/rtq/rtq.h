@@ -295,8 +295,9 @@ typedef struct rtq_tune_result {
     uint32_t from_cache;
 } rtq_tune_result;
 
-/* Times candidate launch settings (refill, chunk, speculation slack, node
- * layout) on the given batch, `reps` launches each, applies the fastest to the context and records
+/* Times candidate launch settings (refill, chunk, node layout, then the
+ * speculation and leaf slacks) on the given batch, `reps` launches each, times
+ * the four fastest again in three rounds, applies the fastest to the context and records
  * it under a key of (device, scene sizes, ray counts, modes); a later call
  * with the same key reuses it without timing.  Writes the items' hit buffers. */
 rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, uint32_t reps,

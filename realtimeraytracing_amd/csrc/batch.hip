@@ -199,6 +199,7 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
     rtq_launch_params best = before;
     float best_ms = base_ms;
     uint32_t tried = 1;
+    std::vector<std::pair<float, rtq_launch_params>> seen{{base_ms, before}};
     // stage 1: refill x chunk x node layout with the slacks in force; stage 2:
     // both slacks with the best stage-1 settings
     auto consider = [&](const rtq_launch_params& p) -> rtq_status {
@@ -207,6 +208,7 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
         const rtq_status s = time_params(p, &ms);
         if (s != RTQ_OK) return s;
         ++tried;
+        seen.emplace_back(ms, p);
         if (ms < best_ms) {
             best_ms = ms;
             best = p;
@@ -221,12 +223,32 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
                 if (status == RTQ_OK)
                     status = consider(rtq_launch_params{rf, ch, wide, before.spec_slack, before.leaf_slack, 0});
     const rtq_launch_params stage1 = best;
-    const uint32_t spec[] = {0, 4, 8, 16};
-    const uint32_t leaf[] = {0, 8, 16};
+    const uint32_t spec[] = {0, 4, 8, 16, 24, 32};
+    const uint32_t leaf[] = {0, 8, 16, 24};
     for (uint32_t sl : spec)
         for (uint32_t ll : leaf)
             if (status == RTQ_OK && !(sl == stage1.spec_slack && ll == stage1.leaf_slack))
                 status = consider(rtq_launch_params{stage1.refill, stage1.chunk, stage1.prefer_wide, sl, ll, 0});
+    // stage 3: one timing of a ~1 ms launch spreads by a few percent, enough
+    // to misorder close settings, so the four fastest run again in three
+    // interleaved rounds and each keeps its lowest time
+    if (status == RTQ_OK) {
+        std::stable_sort(seen.begin(), seen.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        const size_t k = std::min<size_t>(4, seen.size());
+        std::vector<float> low(k, INFINITY);
+        for (int round = 0; round < 3 && status == RTQ_OK; ++round)
+            for (size_t i = 0; i < k && status == RTQ_OK; ++i) {
+                float ms = 0.0f;
+                status = time_params(seen[i].second, &ms);
+                low[i] = std::min(low[i], ms);
+            }
+        if (status == RTQ_OK) {
+            const size_t w = (size_t)(std::min_element(low.begin(), low.end()) - low.begin());
+            best = seen[w].second;
+            best_ms = low[w];
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (status != RTQ_OK) {

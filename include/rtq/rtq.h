@@ -284,6 +284,8 @@ typedef struct rtq_launch_params {
     uint32_t reserved;     /* 0 */
 } rtq_launch_params;
 
+/* A new context starts with refill 32, chunk 128, prefer_wide 1, spec_slack 16,
+ * leaf_slack 8. */
 rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out);
 rtq_status rtq_launch_params_set(rtq_ctx* ctx, const rtq_launch_params* p);
 

@@ -287,7 +287,7 @@ class Context:
         self.check(self.lib.rtq_launch_params_get(self.h, ctypes.byref(p)))
         return p
 
-    def set_launch_params(self, refill: int, chunk: int, prefer_wide: bool = True, spec_slack: int = 4,
+    def set_launch_params(self, refill: int, chunk: int, prefer_wide: bool = True, spec_slack: int = 16,
                           leaf_slack: int = 8) -> None:
         p = LaunchParams(refill, chunk, int(prefer_wide), spec_slack, leaf_slack, 0)
         self.check(self.lib.rtq_launch_params_set(self.h, ctypes.byref(p)))

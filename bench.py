@@ -290,14 +290,20 @@ def main() -> int:
             if form == "learned":
                 learned_setup = train_order(ctx, wl, sp)
                 extra = RTQ_TRACE_LEARNED
-                counters_by_layout["learned"] = wl.counters(RTQ_TRACE_LEARNED, 64)
             items = ctx.items([(wl.sc, wl.diff.data_ptr(), wl.n_diff, wl.hd.data_ptr(), extra),
                                (wl.sc, wl.shad.data_ptr(), wl.n_shad, wl.hs.data_ptr(), RTQ_TRACE_ANY | extra)])
             tr = ctx.autotune(items, reps=2)
             tuned = {"refill": tr.best.refill, "chunk": tr.best.chunk, "prefer_wide": tr.best.prefer_wide,
                      "spec_slack": tr.best.spec_slack, "leaf_slack": tr.best.leaf_slack, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
                      "default_ms": round(tr.default_ms, 4)}
-            lname = "learned" if form == "learned" else ("wide" if tr.best.prefer_wide else "binary")
+            if form == "learned" and wl.sc.info().order_modes:
+                # learned decisions: the batch traces binary nodes with the learned order
+                lname = "learned"
+                counters_by_layout["learned"] = wl.counters(RTQ_TRACE_LEARNED | RTQ_TRACE_BINARY, 64)
+            else:
+                # the batched form, or learned order without decisions (the plain
+                # order), on the layout the autotuner chose
+                lname = "wide" if tr.best.prefer_wide else "binary"
             graph = ctx.batch(items)
             ms_per_step, evs = time_steps(graph_step(graph))
             graph.destroy()

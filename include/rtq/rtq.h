@@ -150,7 +150,8 @@ typedef struct rtq_bvh_info {
     uint32_t node_width;         /* layout traced by default: 2 or 8 */
     uint32_t n_wide_nodes;       /* compressed 8-wide nodes (0 unless node_width == 8) */
     uint32_t wide_depth;         /* levels of the 8-wide tree */
-    uint32_t reserved;
+    uint32_t order_modes;        /* learned child order in the nodes (rtq_order_apply):
+                                    bit 0 closest-hit decisions, bit 1 any-hit decisions */
 } rtq_bvh_info;
 
 rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out);
@@ -213,7 +214,8 @@ rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t
                                        built with node_width 8 */
 #define RTQ_TRACE_LEARNED     0x8u  /* binary layout, child order at nodes where both
                                        children are hit taken from the order bits that
-                                       rtq_order_apply stored (hits are unchanged) */
+                                       rtq_order_apply stored (hits are unchanged); a
+                                       mode without decisions traces as plain rays do */
 
 /* Trace n_rays rays against the scene's BVH.  Replaces the traversal of
  * getClosestHitBVH (raytracer.glsl:246-295) dispatched by drawOneFrame

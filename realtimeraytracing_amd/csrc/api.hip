@@ -508,8 +508,8 @@ rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out) 
     out->build_ms = s->build_ms;
     out->node_width = s->params.node_width;
     out->n_wide_nodes = s->n_wide;
+    out->order_modes = s->ordered ? ((s->order_closest ? 1u : 0u) | (s->order_any ? 2u : 0u)) : 0u;
     out->wide_depth = s->wide_depth;
-    out->reserved = 0;
     return RTQ_OK;
 }
 

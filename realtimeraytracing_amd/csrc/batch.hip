@@ -30,8 +30,10 @@ uint64_t tune_key(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n) {
     h = fnv1a(&ctx->n_cu, sizeof ctx->n_cu, h);
     for (uint32_t i = 0; i < n; ++i) {
         const Scene* s = find_scene(ctx, items[i].scene_id);
-        const uint64_t rec[4] = {s ? s->n_tris : 0u, s ? (uint64_t)(s->wide != nullptr) : 0u, items[i].n_rays,
-                                 items[i].flags};
+        // learned decisions decide the layout of learned-order items, so they key the settings too
+        const uint64_t order = s && s->ordered ? (s->order_closest ? 1u : 0u) | (s->order_any ? 2u : 0u) : 0u;
+        const uint64_t rec[5] = {s ? s->n_tris : 0u, s ? (uint64_t)(s->wide != nullptr) : 0u, items[i].n_rays,
+                                 items[i].flags, order};
         h = fnv1a(rec, sizeof rec, h);
     }
     return h;

@@ -782,17 +782,24 @@ rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, 
 
 // Layout traced for a scene and trace flags; kNone-like -1 with an error set
 // when RTQ_TRACE_LEARNED asks for order bits the scene does not hold.
-int pick_layout(rtq_ctx* ctx, const Scene* s, uint32_t flags) {
+// Learned decisions for the ray's mode: the binary kernel with learned order.
+bool learned_decisions(const Scene* s, uint32_t flags) {
+    return (flags & RTQ_TRACE_LEARNED) && s->ordered && ((flags & RTQ_TRACE_ANY) ? s->order_any : s->order_closest);
+}
+
+// Node layout of a trace.  A learned-order ray without decisions for its mode
+// takes the nearer child everywhere, which is the plain traversal, on the
+// layout plain rays use; allow_wide false keeps it (and plain rays) on binary
+// nodes, for batches that also hold learned-order rays.
+int pick_layout(rtq_ctx* ctx, const Scene* s, uint32_t flags, bool allow_wide = true) {
     if (flags & RTQ_TRACE_LEARNED) {
         if (!s->ordered) {
             fail(ctx, RTQ_ENOTBUILT, "trace: RTQ_TRACE_LEARNED needs order bits (rtq_order_apply)");
             return -1;
         }
-        // a mode without decisions takes the nearer child everywhere: the plain kernel
-        const bool decisions = (flags & RTQ_TRACE_ANY) ? s->order_any : s->order_closest;
-        return decisions ? kOrdered : kBinary;
+        if (learned_decisions(s, flags)) return kOrdered;
     }
-    return (s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide) ? kWide : kBinary;
+    return (allow_wide && s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide) ? kWide : kBinary;
 }
 
 }  // namespace
@@ -868,6 +875,13 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
     uint64_t total = 0, depth = 0;
     const uint32_t leaf_flags = items[0].flags & RTQ_TRACE_LEAF_REFMT;
     int layout = -1;
+    // items with learned decisions trace binary nodes, and so does the rest of
+    // the batch (plain and learned binary items share one kernel)
+    bool any_ordered = false;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const Scene* s = find_scene(ctx, items[i].scene_id);
+        if (s && s->built && learned_decisions(s, items[i].flags)) any_ordered = true;
+    }
     for (uint32_t i = 0; i < n_items; ++i) {
         const rtq_batch_item& it = items[i];
         const Scene* s = find_scene(ctx, it.scene_id);
@@ -879,7 +893,7 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
             return fail(ctx, RTQ_EINVAL, "trace_batch: all items must use the same leaf test");
         if (it.n_rays && (!it.rays_dev || !it.hits_dev))
             return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: null buffer", i);
-        const int lay = pick_layout(ctx, s, it.flags);
+        const int lay = pick_layout(ctx, s, it.flags, !any_ordered);
         if (lay < 0) return RTQ_ENOTBUILT;
         const bool wide = lay == kWide;
         // binary items with and without learned order share the ordered kernel

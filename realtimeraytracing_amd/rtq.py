@@ -88,7 +88,7 @@ class BvhInfo(ctypes.Structure):
                 ("ploc_iterations", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
                 ("box_pad", ctypes.c_float), ("build_ms", ctypes.c_float),
                 ("node_width", ctypes.c_uint32), ("n_wide_nodes", ctypes.c_uint32),
-                ("wide_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("wide_depth", ctypes.c_uint32), ("order_modes", ctypes.c_uint32)]
 
 
 class TrainParams(ctypes.Structure):

@@ -270,11 +270,21 @@ def test_learned_order_exact(rtq_ctx, occ, torch_mod):
     rtq_ctx.synchronize()
     assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d)
     assert np.array_equal(devmem.to_host(hs, HIT_DTYPE)["id"] >= 0, plain_s["id"] >= 0)
-    # ... but not with an 8-wide item
+    # with learned decisions in the batch, a scene built with 8-wide nodes is
+    # traced on its binary nodes in the same launch
     wide = rtq_ctx.upload(*scenes.soup_scene(2000, 3))
     wide.build(16, 4, 0, node_width=8)
     try:
-        bad = rtq_ctx.items([(sc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
+        assert sc.info().order_modes != 0 and wide.info().order_modes == 0
+        hw = devmem.empty(1000, HIT_DTYPE)
+        both = rtq_ctx.items([(sc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
+                              (wide, occ["diff"].data_ptr(), 1000, hw.data_ptr(), 0)])
+        rtq_ctx.trace_batch(both)
+        rtq_ctx.synchronize()
+        assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d)
+        assert hits_equal(devmem.to_host(hw, HIT_DTYPE), trace(wide, occ["diff"], 1000, 0))
+        # explicit binary items and 8-wide items cannot share a launch
+        bad = rtq_ctx.items([(wide, occ["diff"].data_ptr(), 1000, hw.data_ptr(), RTQ_TRACE_BINARY),
                              (wide, occ["diff"].data_ptr(), 1000, hs.data_ptr(), 0)])
         with pytest.raises(RtqError):
             rtq_ctx.trace_batch(bad)

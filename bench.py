@@ -204,8 +204,8 @@ def t_error_stats(tris, rays, hits, n_sample=20_000, seed=0):
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU only; "
-                                   "the trace path shards by ray ranges with a replicated BVH (see DESIGN.md)"}))
+        print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU only; the trace path "
+                                   "would shard by ray ranges over a BVH replicated on every GPU"}))
         return 2
     import torch
     from realtimeraytracing_amd import (RTQ_BUILD_PLOC_SPLIT, RTQ_TRACE_ANY, RTQ_TRACE_BINARY, RTQ_TRACE_LEARNED, Context, devmem,

@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--no-shade", action="store_true", help="skip shading / BVH-view timing")
     ap.add_argument("--no-refit", action="store_true", help="skip the scene-edit refit timing")
     ap.add_argument("--no-build-timing", action="store_true", help="skip the fused / split build timing")
+    ap.add_argument("--batched-layout", choices=("tuned", "binary", "wide"), default="tuned",
+                    help="node layout of the batched form: the autotuner's choice, or pinned (profiling one kernel)")
     ap.add_argument("--no-cpu-reference-case", action="store_true",
                     help="skip the 65,536-triangle soup against the CPU reference traversal")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
@@ -293,6 +295,10 @@ def main() -> int:
             items = ctx.items([(wl.sc, wl.diff.data_ptr(), wl.n_diff, wl.hd.data_ptr(), extra),
                                (wl.sc, wl.shad.data_ptr(), wl.n_shad, wl.hs.data_ptr(), RTQ_TRACE_ANY | extra)])
             tr = ctx.autotune(items, reps=2)
+            if form == "batched" and args.batched_layout != "tuned":
+                tr.best.prefer_wide = int(args.batched_layout == "wide")
+                ctx.set_launch_params(tr.best.refill, tr.best.chunk, bool(tr.best.prefer_wide), tr.best.spec_slack,
+                                      tr.best.leaf_slack)
             tuned = {"refill": tr.best.refill, "chunk": tr.best.chunk, "prefer_wide": tr.best.prefer_wide,
                      "spec_slack": tr.best.spec_slack, "leaf_slack": tr.best.leaf_slack, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
                      "default_ms": round(tr.default_ms, 4)}

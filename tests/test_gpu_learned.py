@@ -362,3 +362,37 @@ def test_order_model_handle(rtq_ctx, occ, tmp_path):
     finally:
         m.destroy()
         sc.order_apply(None)
+
+
+def test_learned_without_decisions_traces_like_plain_on_wide_nodes(rtq_ctx, torch_mod):
+    """Order words applied with both modes turned off (confidence 1): learned
+    rays trace as plain rays do, on the scene's 8-wide nodes, bit for bit; with
+    any-hit decisions on, closest-hit rays still match and a batch of both
+    modes runs on binary nodes with the same results."""
+    torch = torch_mod
+    tris, models = scenes.soup_scene(20000, 4)
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        sc.build(16, 4, 0, node_width=8)
+        rays = devmem.to_device(scenes.random_rays(8192, 6, (-0.1, -0.1, -0.1), (1.1, 1.1, 1.1)))
+        n = 8192
+        plain_c, plain_a = trace(sc, rays, n, 0), trace(sc, rays, n, RTQ_TRACE_ANY)
+        params = dev(rand_params(21), torch)
+        torch.cuda.synchronize()
+        sc.order_apply(params.data_ptr(), 1.0, 1.0)
+        assert sc.info().order_modes == 0
+        assert hits_equal(trace(sc, rays, n, RTQ_TRACE_LEARNED), plain_c)
+        assert hits_equal(trace(sc, rays, n, RTQ_TRACE_ANY | RTQ_TRACE_LEARNED), plain_a)
+        sc.order_apply(params.data_ptr(), 0.5, 1.0)
+        assert sc.info().order_modes == 2
+        assert hits_equal(trace(sc, rays, n, RTQ_TRACE_LEARNED), plain_c)
+        got_a = trace(sc, rays, n, RTQ_TRACE_ANY | RTQ_TRACE_LEARNED)
+        assert np.array_equal(got_a["id"] >= 0, plain_a["id"] >= 0)
+        hc, ha = devmem.empty(n, HIT_DTYPE), devmem.empty(n, HIT_DTYPE)
+        rtq_ctx.trace_batch(rtq_ctx.items([(sc, rays.data_ptr(), n, hc.data_ptr(), RTQ_TRACE_LEARNED),
+                                           (sc, rays.data_ptr(), n, ha.data_ptr(), RTQ_TRACE_ANY | RTQ_TRACE_LEARNED)]))
+        rtq_ctx.synchronize()
+        assert hits_equal(devmem.to_host(hc, HIT_DTYPE), plain_c)
+        assert np.array_equal(devmem.to_host(ha, HIT_DTYPE)["id"] >= 0, plain_a["id"] >= 0)
+    finally:
+        sc.release()

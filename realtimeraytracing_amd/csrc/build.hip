@@ -1230,7 +1230,7 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     // size the context's scratch for this build's temporaries up front (they
     // come to about 120 B per triangle: sort and PLOC lists 64, placement 40,
     // 8-wide collapse 16), so a first build does not allocate them one by one
-    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, (size_t)128 * n + ((size_t)1 << 20));
+    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, (size_t)160 * n + ((size_t)1 << 20));
     rtq_status rs = arena_reserve(ctx);
     if (rs != RTQ_OK) return rs;
 

@@ -1,14 +1,11 @@
 // Compressed 8-wide traversal nodes built from the binary PLOC tree.
 //
-// Collapse: top-down, one level per launch.  A wide node starts from the two
-// children of its binary node and repeatedly replaces the child with the
-// largest surface area that is still "expandable" (internal with more than
-// max_leaf triangles) by that child's two children, until it has 8 children
-// or none is expandable (the greedy collapse of Wald et al., "Getting Rid of
-// Packets", 2008).  Expandable children become wide nodes of the next level;
-// their indices come from a scan, so numbering is deterministic and the
-// children of a node are contiguous.  Other children are leaves: the
-// subtree's triangles are already contiguous in leaf order.
+// Collapse: top-down, one level per launch.  Each internal cluster's 8 slots
+// are given out as an SAH cost table says (see k_dp_pass below): a child that
+// is internal with more than max_leaf triangles becomes a wide node of the
+// next level, any other child a leaf (the subtree's triangles are already
+// contiguous in leaf order).  Next-level node indices come from a scan, so
+// numbering is deterministic and the children of a node are contiguous.
 //
 // Node (128 B, 8 float4; only the first 96 B are read by the traversal):
 //   [ox oy oz meta] meta = ex | ey << 8 | ez << 16 | valid << 24
@@ -39,11 +36,98 @@ __device__ __forceinline__ float box_area(const rtq_aabb& b) {
     return 2.0f * ((dx * dy + dy * dz) + dz * dx);
 }
 
+// SAH-optimal collapse (Ylitie, Karras & Laine, "Efficient incoherent ray
+// traversal on GPUs through compressed wide BVHs", HPG 2017), restricted to
+// this layout: cost[x][j-1] is the least SAH cost of cluster x's subtree when
+// it may fill j slots of its parent: a cluster of more than max_leaf triangles
+// given one slot is an 8-wide node (node cost + the best split of 8 slots
+// between its children), a smaller one a leaf (triangle cost x count); j > 1
+// slots go either to the cluster with fewer slots or split between its
+// children.  choice[x] keeps, per j = 2..8, 0 (fewer slots) or the left
+// child's share (bits 3(j-2)), and the node's own split (bits 21..23).
+// Rows are computed bottom-up, one PLOC pass at a time (a merge's children
+// come from earlier passes); single triangles have a constant row.
+constexpr float kNodeCost = 1.0f;
+constexpr float kTriCost = 0.6f;  // best of 0.15-4 on the 2M-triangle terrain
+
+__device__ __forceinline__ void dp_row(uint32_t x, uint32_t nt, const float* __restrict__ cost,
+                                       const rtq_aabb* __restrict__ cbox, float* row) {
+    if (x < nt) {
+        const float a = kTriCost * box_area(cbox[x]);
+#pragma unroll
+        for (int j = 0; j < kWidth; ++j) row[j] = a;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kWidth; ++j) row[j] = cost[(size_t)(x - nt) * kWidth + j];
+    }
+}
+
+__device__ __forceinline__ void dp_cluster(uint32_t c, uint32_t nt, uint32_t max_leaf,
+                                           const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
+                                           const uint32_t* __restrict__ ccount, const rtq_aabb* __restrict__ cbox,
+                                           float* __restrict__ cost, uint32_t* __restrict__ choice) {
+    float dl[kWidth], dr[kWidth], row[kWidth];
+    dp_row(cleft[c], nt, cost, cbox, dl);
+    dp_row(cright[c], nt, cost, cbox, dr);
+    float bn = INFINITY;
+    uint32_t kn = 1;
+    for (uint32_t k = 1; k < kWidth; ++k) {
+        const float v = dl[k - 1] + dr[kWidth - k - 1];
+        if (v < bn) { bn = v; kn = k; }
+    }
+    const float a = box_area(cbox[c]);
+    const uint32_t cnt = ccount[c];
+    row[0] = cnt <= max_leaf ? kTriCost * a * (float)cnt : kNodeCost * a + bn;
+    uint32_t ch = (kn - 1) << 21;
+    for (uint32_t j = 2; j <= kWidth; ++j) {
+        float best = row[j - 2];
+        uint32_t pick = 0;
+        for (uint32_t k = 1; k < j; ++k) {
+            const float v = dl[k - 1] + dr[j - k - 1];
+            if (v < best) { best = v; pick = k; }
+        }
+        row[j - 1] = best;
+        ch |= pick << (3 * (j - 2));
+    }
+    float* out = cost + (size_t)(c - nt) * kWidth;
+#pragma unroll
+    for (int j = 0; j < kWidth; ++j) out[j] = row[j];
+    choice[c - nt] = ch;
+}
+
+__global__ __launch_bounds__(kThreads) void k_dp_pass(uint32_t first, uint32_t count, uint32_t nt, uint32_t max_leaf,
+                                                      const uint32_t* __restrict__ cleft,
+                                                      const uint32_t* __restrict__ cright,
+                                                      const uint32_t* __restrict__ ccount,
+                                                      const rtq_aabb* __restrict__ cbox, float* __restrict__ cost,
+                                                      uint32_t* __restrict__ choice) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < count) dp_cluster(first + i, nt, max_leaf, cleft, cright, ccount, cbox, cost, choice);
+}
+
+// Passes [k0, k1) in one workgroup, a barrier between passes.
+constexpr int kDpSmallThreads = 1024;
+__global__ __launch_bounds__(kDpSmallThreads) void k_dp_small(const uint2* __restrict__ passes, uint32_t k0,
+                                                              uint32_t k1, uint32_t nt, uint32_t max_leaf,
+                                                              const uint32_t* __restrict__ cleft,
+                                                              const uint32_t* __restrict__ cright,
+                                                              const uint32_t* __restrict__ ccount,
+                                                              const rtq_aabb* __restrict__ cbox,
+                                                              float* __restrict__ cost, uint32_t* __restrict__ choice) {
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint2 p = passes[k];
+        for (uint32_t i = threadIdx.x; i < p.y; i += kDpSmallThreads)
+            dp_cluster(p.x + i, nt, max_leaf, cleft, cright, ccount, cbox, cost, choice);
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t n_items, const uint32_t* __restrict__ items,
                                                             uint32_t max_leaf, const uint32_t* __restrict__ cleft,
                                                             const uint32_t* __restrict__ cright,
                                                             const uint32_t* __restrict__ ccount,
                                                             const rtq_aabb* __restrict__ cbox,
+                                                            uint32_t nt, const uint32_t* __restrict__ choice,
                                                             uint32_t* __restrict__ children,
                                                             uint32_t* __restrict__ n_children,
                                                             uint32_t* __restrict__ n_expand) {
@@ -55,36 +139,26 @@ __global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t n_items, co
     if (cleft[c] == kNone) {
         ch[n++] = c;  // single-triangle scene: the root itself is the only (leaf) child
     } else {
-        ch[n++] = cleft[c];
-        ch[n++] = cright[c];
-        while (n < kWidth) {
-            int best = -1;
-            float best_a = -1.0f;
-            for (uint32_t i = 0; i < n; ++i) {
-                if (!expandable(ch[i], max_leaf, cleft, ccount)) continue;
-                const float a = box_area(cbox[ch[i]]);
-                if (a > best_a) { best_a = a; best = (int)i; }
+        // the slots as the cost table splits them, children left to right
+        const uint32_t kn = ((choice[c - nt] >> 21) & 7u) + 1u;
+        uint32_t sx[kWidth], sj[kWidth];
+        int sp = 0;
+        sx[sp] = cright[c]; sj[sp++] = kWidth - kn;
+        sx[sp] = cleft[c]; sj[sp++] = kn;
+        while (sp > 0) {
+            --sp;
+            const uint32_t x = sx[sp], j = sj[sp];
+            if (j == 1 || cleft[x] == kNone) {
+                ch[n++] = x;
+                continue;
             }
-            if (best < 0) break;
-            const uint32_t e = ch[best];
-            ch[best] = cleft[e];
-            ch[n++] = cright[e];
-        }
-        // slots left over once every child is leaf-sized: split the largest
-        // leaf-sized clusters into smaller leaves (a visit tests all 8 slots
-        // anyway, and smaller leaf boxes cull more triangle tests)
-        while (n < kWidth) {
-            int best = -1;
-            float best_a = -1.0f;
-            for (uint32_t i = 0; i < n; ++i) {
-                if (cleft[ch[i]] == kNone) continue;
-                const float a = box_area(cbox[ch[i]]);
-                if (a > best_a) { best_a = a; best = (int)i; }
+            const uint32_t pick = (choice[x - nt] >> (3 * (j - 2))) & 7u;
+            if (pick == 0) {
+                sx[sp] = x; sj[sp++] = j - 1;
+            } else {
+                sx[sp] = cright[x]; sj[sp++] = j - pick;
+                sx[sp] = cleft[x]; sj[sp++] = pick;
             }
-            if (best < 0) break;
-            const uint32_t e = ch[best];
-            ch[best] = cleft[e];
-            ch[n++] = cright[e];
         }
     }
     uint32_t ne = 0;
@@ -201,11 +275,32 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st
     if (!s->wide) RTQ_HIP(ctx, dev_alloc(&s->wide, 8ull * cap));
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, items, root);
     RTQ_HIP(ctx, hipGetLastError());
+    // cost table of the collapse, bottom-up over the PLOC passes
+    const uint32_t nt = s->n_tris, max_leaf = s->params.max_leaf_size;
+    uint32_t* choice = nullptr;
+    if (nc > nt) {
+        float* cost = nullptr;
+        RTQ_HIP(ctx, tmp.get(&cost, (size_t)(nc - nt) * kWidth));
+        RTQ_HIP(ctx, tmp.get(&choice, nc - nt));
+        const uint32_t n_passes = (uint32_t)s->passes.size();
+        for (uint32_t k = 0; k < s->small_from; ++k) {
+            const auto& pr = s->passes[k];
+            hipLaunchKernelGGL(k_dp_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second, nt,
+                               max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
+            RTQ_HIP(ctx, hipGetLastError());
+        }
+        if (s->small_from < n_passes) {
+            hipLaunchKernelGGL(k_dp_small, dim3(1), dim3(kDpSmallThreads), 0, st, s->d_passes, s->small_from, n_passes,
+                               nt, max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
+            RTQ_HIP(ctx, hipGetLastError());
+        }
+    }
     uint32_t level_items = 1, level_base = 0, levels = 0;
     while (level_items > 0) {
         if (level_base + level_items > cap) return fail(ctx, RTQ_EINTERNAL, "wide collapse overflow");
         hipLaunchKernelGGL(k_wide_collapse, dim3(blocks_for(level_items)), dim3(kThreads), 0, st, level_items, items,
-                           s->params.max_leaf_size, s->cleft, s->cright, s->ccount, s->cbox, children, nchild, nexp);
+                           s->params.max_leaf_size, s->cleft, s->cright, s->ccount, s->cbox, nt, choice, children,
+                           nchild, nexp);
         RTQ_HIP(ctx, hipGetLastError());
         rtq_status rs = scan_u32(ctx, nexp, offs, level_items, total, st);
         if (rs != RTQ_OK) return rs;

@@ -556,6 +556,9 @@ def main() -> int:
     if not args.no_compressed_8m:
         from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE
         t4 = time.time()
+        # an earlier block's autotune may have left binary nodes preferred
+        ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
+                              default_params.spec_slack, default_params.leaf_slack)
         ttris = scenes.tiled_terrains(1024, 2, seed=0)
         tsc = ctx.upload(ttris, scenes.identity_models())
         tinfo = tsc.build(16, args.max_leaf, 0, node_width=8)

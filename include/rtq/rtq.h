@@ -303,7 +303,8 @@ typedef struct rtq_tune_result {
  * speculation and leaf slacks) on the given batch, `reps` launches each, times
  * the four fastest again in three rounds, applies the fastest to the context and records
  * it under a key of (device, scene sizes, ray counts, modes); a later call
- * with the same key reuses it without timing.  Writes the items' hit buffers. */
+ * with the same key reuses it without timing.  Writes the items' hit buffers.
+ * The applied settings stay in force for later traces on any scene. */
 rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, uint32_t reps,
                         rtq_tune_result* out);
 /* Versioned, checksummed file of the tuning cache. */

@@ -31,9 +31,9 @@ records compared whole, t error against an FP64 ray/plane distance.
 one graph-replayed launch against 64 launches, results compared.
 `detail.occluded` measures the learned-ordering workload: the same terrain
 plus 2,097,152 random-soup triangles over the unit cube, 2048 x 2048 primary
-hits -> 4M diffuse + 16M shadow rays (nearly all occluded): plain, learned
-order (two launches) and learned order batched in one graph-replayed launch,
-with the same exactness check.
+hits -> 4M diffuse + 16M shadow rays (nearly all occluded): plain, 8-wide
+compressed nodes, learned order (two launches) and learned order batched in
+one graph-replayed launch, with the same exactness check.
 
 Prints one JSON line with the metric, a roofline object for the dominant
 trace kernel of the best form (algorithmic bytes per launch from the
@@ -504,17 +504,19 @@ def main() -> int:
         otris, omodels = scenes.occluded_scene(1024, 2_097_152, seed=0)
         ocam = scenes.pinhole(**{**scenes.TERRAIN_CAMERA, "width": args.occluded_res,
                                  "height": args.occluded_res})
-        owl = Workload(ctx, otris, omodels, ocam, sp, args.max_leaf, node_width=2)
+        ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
+                              default_params.spec_slack, default_params.leaf_slack)
+        owl = Workload(ctx, otris, omodels, ocam, sp, args.max_leaf, node_width=8)
         n_otris = len(otris)
         del otris
         oforms = {}
         oref = None
-        for form in ("plain", "learned", "learned_batched"):
-            flags = RTQ_TRACE_BINARY
+        for form in ("plain", "compressed", "learned", "learned_batched"):
+            flags = 0 if form == "compressed" else RTQ_TRACE_BINARY
             setup = None
             if form == "learned":
                 setup = train_order(ctx, owl, sp)
-            if form != "plain":
+            if form.startswith("learned"):
                 flags = RTQ_TRACE_LEARNED
             if form == "learned_batched":
                 ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
@@ -531,7 +533,7 @@ def main() -> int:
             got = owl.hits()
             if oref is None:
                 oref = got
-            c = owl.counters(flags, 64)
+            c = owl.counters(flags, 128 if form == "compressed" else 64)
             split = form != "learned_batched"
             oforms[form] = {
                 "Mrays_s": round(owl.live_rays / (ms_per_step * 1e-3) / 1e6, 2),
@@ -546,6 +548,7 @@ def main() -> int:
                         f"{args.occluded_res}x{args.occluded_res} primary hits -> 1 diffuse + 4 shadow rays each",
             "rays_per_step": owl.live_rays, "forms": oforms,
             "learned_vs_plain": round(oforms["learned"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
+            "compressed_vs_plain": round(oforms["compressed"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
             "learned_batched_vs_plain": round(oforms["learned_batched"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
             "shadow_speedup": round(oforms["plain"]["shadow_ms"] / oforms["learned"]["shadow_ms"], 3),
             "setup_s": round(time.time() - t3, 1)}

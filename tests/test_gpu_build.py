@@ -100,6 +100,35 @@ def test_fused_and_split_ploc_agree(rtq_ctx, n, radius):
         sc.release()
 
 
+@pytest.mark.parametrize("n", [9000, 200_000])
+def test_split_schedule_8wide_nodes(rtq_ctx, n):
+    """The 8-wide collapse reads the PLOC passes the build recorded: both
+    schedules give the same 8-wide tree, and its hits equal the binary ones."""
+    from realtimeraytracing_amd import RAY_DTYPE, RTQ_TRACE_ANY, RTQ_TRACE_BINARY
+    tris, models = _scene(n, seed=n + 3)
+    rng = np.random.default_rng(n)
+    rays = np.zeros(30_000, RAY_DTYPE)
+    rays["o"] = rng.uniform(-0.2, 1.2, (len(rays), 3)).astype(np.float32)
+    d = rng.normal(size=(len(rays), 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmax"] = np.inf
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        out = []
+        for flags in (0, RTQ_BUILD_PLOC_SPLIT):
+            info = sc.build(16, 4, flags, node_width=8)
+            assert info.node_width == 8
+            wide, binary = sc.trace_host(rays), sc.trace_host(rays, RTQ_TRACE_BINARY)
+            assert hits_equal(wide, binary), first_diff(wide, binary)
+            occ_w, occ_b = sc.trace_host(rays, RTQ_TRACE_ANY), sc.trace_host(rays, RTQ_TRACE_ANY | RTQ_TRACE_BINARY)
+            assert np.array_equal(occ_w["id"] >= 0, occ_b["id"] >= 0)
+            out.append((info.n_wide_nodes, info.wide_depth, wide))
+        assert out[0][:2] == out[1][:2]
+        assert hits_equal(out[0][2], out[1][2])
+    finally:
+        sc.release()
+
+
 @pytest.mark.parametrize("kind", ["copies", "identical"])
 def test_fused_ploc_with_ties(rtq_ctx, kind):
     """Equal Morton codes and equal merged areas across tile boundaries of the

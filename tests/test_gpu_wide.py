@@ -93,3 +93,82 @@ def test_wide_node_width_validation(rtq_ctx):
             sc.build(16, 4, 0, node_width=4)
     finally:
         sc.release()
+
+
+def _collapse_restated(cl, n_tris, max_leaf, tri_cost=np.float32(0.6)):
+    """CPU restatement of the 8-wide collapse: the SAH cost table over slot
+    counts (float32, the device's operation order), then the top-down
+    extraction.  Returns (wide nodes, levels)."""
+    f32, none = np.float32, 0xFFFFFFFF
+    left, right, boxes = cl["left"], cl["right"], cl["boxes"]
+    nc = 2 * n_tris - 1
+    count = np.ones(nc, np.int64)
+    for c in range(n_tris, nc):
+        count[c] = count[left[c]] + count[right[c]]
+
+    def area(c):
+        lo, hi = boxes["min"][c], boxes["max"][c]
+        dx, dy, dz = f32(hi[0] - lo[0]), f32(hi[1] - lo[1]), f32(hi[2] - lo[2])
+        return f32(2.0) * f32(f32(f32(dx * dy) + f32(dy * dz)) + f32(dz * dx))
+
+    rows, choice = {}, {}
+
+    def row(x):
+        return [f32(tri_cost * area(x))] * 8 if x < n_tris else rows[x]
+
+    for c in range(n_tris, nc):
+        dl, dr = row(left[c]), row(right[c])
+        bn, kn = f32(np.inf), 1
+        for k in range(1, 8):
+            v = f32(dl[k - 1] + dr[8 - k - 1])
+            if v < bn:
+                bn, kn = v, k
+        a = area(c)
+        r = [f32(f32(tri_cost * a) * f32(count[c])) if count[c] <= max_leaf else f32(a + bn)]
+        picks = []
+        for j in range(2, 9):
+            best, pick = r[j - 2], 0
+            for k in range(1, j):
+                v = f32(dl[k - 1] + dr[j - k - 1])
+                if v < best:
+                    best, pick = v, k
+            r.append(best)
+            picks.append(pick)
+        rows[c], choice[c] = r, (kn, picks)
+
+    def children(c):
+        if left[c] == none:
+            return [c]
+        kn = choice[c][0]
+        out, stack = [], [(right[c], 8 - kn), (left[c], kn)]
+        while stack:
+            x, j = stack.pop()
+            if j == 1 or left[x] == none:
+                out.append(x)
+                continue
+            pick = choice[x][1][j - 2]
+            stack += [(x, j - 1)] if pick == 0 else [(right[x], j - pick), (left[x], pick)]
+        assert len(out) <= 8
+        return out
+
+    items, n_nodes, levels = [nc - 1], 0, 0
+    while items:
+        n_nodes += len(items)
+        items = [x for c in items for x in children(c) if left[x] != none and count[x] > max_leaf]
+        levels += 1
+    return n_nodes, levels
+
+
+@pytest.mark.parametrize("n,leaf", [(1500, 2), (3000, 4)])
+def test_wide_collapse_follows_cost_table(rtq_ctx, n, leaf):
+    """Node count and depth of the 8-wide tree equal those of the collapse
+    restated on the CPU from the exported clusters (hits cannot show a wrong
+    cost table: any tree shape gives the same hits)."""
+    tris, models = scenes.soup(n, seed=40 + n), scenes.identity_models()
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        info = sc.build(16, leaf, 0, node_width=8)
+        want = _collapse_restated(sc.export_clusters(), n, leaf)
+        assert (info.n_wide_nodes, info.wide_depth) == want
+    finally:
+        sc.release()

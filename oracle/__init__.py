@@ -59,6 +59,7 @@ def lib() -> ctypes.CDLL:
             "orc_trace_brute": (None, [vp, u32, vp, u64, i32, i32, vp, i32]),
             "orc_trace_ref": (None, [vp, vp, vp, u64, i32, vp, i32]),
             "orc_trace_bvh": (None, [vp, vp, vp, u64, i32, i32, f32, vp, i32, vp, vp]),
+            "orc_gpu_box_check": (None, [vp, u32, vp, vp, u64, f32, f32, vp, vp]),
             "orc_rng_hash": (u64, [u64, u64, u64]),
             "orc_u01f": (f32, [u64, u64, u64]),
             "orc_raygen_pinhole": (None, [vp, vp, vp, vp, u32, u32, vp]),
@@ -217,13 +218,28 @@ def trace_ref(nodes, verts9, rays, leaf=LEAF_REFMT, threads=0):
 def trace_bvh(nodes, verts9, rays, leaf=LEAF_WATERTIGHT, mode=MODE_CLOSEST, pad=None, threads=0):
     TRI, MOD, AABB, NODE, RAY, HIT = _dtypes()
     if pad is None:
-        ext = max(float(np.abs(nodes[0]["box"]["min"]).max()), float(np.abs(nodes[0]["box"]["max"]).max()))
-        pad = max(ext * 2.0 ** -18, 1e-30)
+        pad = scene_pad(nodes)
     hits = np.zeros(len(rays), HIT)
     visits, tests = ctypes.c_uint64(), ctypes.c_uint64()
     lib().orc_trace_bvh(_p(nodes), _p(verts9), _p(rays), len(rays), leaf, mode, pad, _p(hits), _threads(threads),
                         ctypes.byref(visits), ctypes.byref(tests))
     return hits, visits.value, tests.value
+
+
+def scene_pad(nodes) -> float:
+    """The build's box pad: 2^-18 of the largest |coordinate| of the root box."""
+    ext = max(float(np.abs(nodes[0]["box"]["min"]).max()), float(np.abs(nodes[0]["box"]["max"]).max()))
+    return max(ext * 2.0 ** -18, 1e-30)
+
+
+def gpu_box_check(nodes, rays, hits, rel, pad=None):
+    """(rays whose hit a binary box test would skip, the same for 8-wide)."""
+    if pad is None:
+        pad = scene_pad(nodes)
+    lb, lw = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().orc_gpu_box_check(_p(nodes), len(nodes), _p(rays), _p(hits), len(rays), pad, rel, ctypes.byref(lb),
+                            ctypes.byref(lw))
+    return lb.value, lw.value
 
 
 def rng_hash(seed, stream, index) -> int:

@@ -585,23 +585,33 @@ void orc_trace_ref(const rtq_bvh_node* nodes, const float* v9, const rtq_ray* ra
     }
 }
 
-/* Conservative slab test: boxes widened by `pad`, far distance widened by
- * 1 + 2*gamma(3) (Ize 2013 / Pharr et al.), inclusive comparison. */
+/* Relative widening of a far distance: a box is entered while
+ * tn <= tf + |tf| * ORC_SLAB_REL.  Errors proportional to the distance (the
+ * reciprocal, the subtraction of the origin and the product here; the
+ * watertight test's translation of far vertices and its t = T / det) stay
+ * below a sixteenth of it, so with boxes padded by `pad` (errors proportional
+ * to the scene's coordinates) the test keeps every watertight hit for any ray
+ * origin (Ize, "Robust BVH Ray Traversal", JCGT 2013, widens the far distance
+ * the same way).  The same constant widens the GPU kernels' far distances. */
+#define ORC_SLAB_REL 0x1p-19f
+
+static int slab_keep(float tn, float tf) { return tn <= fmaf(fabsf(tf), ORC_SLAB_REL, tf); }
+
+/* Conservative slab test: boxes widened by `pad`, far distance widened
+ * relatively (slab_keep), inclusive comparison. */
 static int cons_slab(const float o[3], const float inv[3], const rtq_aabb* b, float pad,
                      float t0, float t1, float* tnear) {
-    const float expand = 1.0f + 2.0f * (3.0f * 5.9604645e-8f) / (1.0f - 3.0f * 5.9604645e-8f);
     float lo, hi, tn = t0, tf = t1;
     for (int k = 0; k < 3; ++k) {
         float a = ((b->min[k] - pad) - o[k]) * inv[k];
         float c = ((b->max[k] + pad) - o[k]) * inv[k];
         lo = smin(a, c);
         hi = smax(a, c);
-        if (hi > 0.0f) hi = hi * expand;
         tn = smax(tn, lo);
         tf = smin(tf, hi);
     }
     *tnear = tn;
-    return tn <= tf;
+    return slab_keep(tn, tf);
 }
 
 void orc_trace_bvh(const rtq_bvh_node* nodes, const float* v9, const rtq_ray* rays,
@@ -674,7 +684,7 @@ void orc_trace_bvh(const rtq_bvh_node* nodes, const float* v9, const rtq_ray* ra
                 done = 1;
                 while (sp > 0) {
                     --sp;
-                    if (st_t[sp] <= tbest) { cur = st_n[sp]; done = 0; break; }
+                    if (slab_keep(st_t[sp], tbest)) { cur = st_n[sp]; done = 0; break; }
                 }
             }
             hits[r] = best;
@@ -683,6 +693,104 @@ void orc_trace_bvh(const rtq_bvh_node* nodes, const float* v9, const rtq_ray* ra
     }
     if (visits_out) *visits_out = visits;
     if (tests_out) *tests_out = tests;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The GPU kernels' box tests, restated to check them against brute force    */
+
+static float gpu_rcp(float d, int var) {
+    /* v_rcp_f32 is within 1 ulp of 1/d: nearest, or one ulp up or down */
+    float r = 1.0f / d;
+    if (var == 1) r = nextafterf(r, INFINITY);
+    if (var == 2) r = nextafterf(r, -INFINITY);
+    return r;
+}
+
+/* trace.hip, binary visit: plane distance fma(b, inv, -(o * inv)) on boxes
+ * padded by `pad`, entered while tn <= tf + |tf| * rel. */
+static int gpu_box_binary(const rtq_ray* r, int var, const rtq_aabb* b, float pad, float tbest, float rel) {
+    float tn = r->tmin, tf = tbest;
+    for (int k = 0; k < 3; ++k) {
+        float e = fabsf(r->d[k]) < 1e-30f ? copysignf(1e-30f, r->d[k]) : r->d[k];
+        float inv = gpu_rcp(e, var);
+        float oi = r->o[k] * inv;
+        float a = fmaf(b->min[k] - pad, inv, -oi), c = fmaf(b->max[k] + pad, inv, -oi);
+        tn = fmaxf(tn, fminf(a, c));
+        tf = fminf(tf, fmaxf(a, c));
+    }
+    return tn <= fmaf(fabsf(tf), rel, tf);
+}
+
+static int wide_exponent(double extent) {
+    /* wide.hip exponent_for: smallest e with 255 * 2^e >= extent */
+    int e = extent > 0.0 ? ilogb(extent) - 7 : -126;
+    if (e < -126) e = -126;
+    while (e < 127 && ldexp(255.0, e) < extent) ++e;
+    while (e > -126 && ldexp(255.0, e - 1) >= extent) --e;
+    return e;
+}
+
+/* trace.hip / wide.hip, 8-wide visit: the child box quantised outward to 8 bits
+ * in a node frame (here the parent box's padded frame stands in for the 8-wide
+ * node's), planes fma(q, 2^e * inv, fma(origin, inv, -(o * inv))), near and far
+ * plane picked by the sign of inv. */
+static int gpu_box_wide(const rtq_ray* r, int var, const rtq_aabb* b, const rtq_aabb* frame, float pad, float tbest,
+                        float rel) {
+    float tn = r->tmin, tf = tbest;
+    for (int k = 0; k < 3; ++k) {
+        float org = frame->min[k] - pad;
+        int ex = wide_exponent((double)(frame->max[k] + pad) - (double)org);
+        double scale = ldexp(1.0, ex);
+        double l = floor(((double)(b->min[k] - pad) - (double)org) / scale);
+        double h = ceil(((double)(b->max[k] + pad) - (double)org) / scale);
+        l = fmin(fmax(l, 0.0), 255.0);
+        h = fmin(fmax(h, 0.0), 255.0);
+        float e = fabsf(r->d[k]) < 1e-30f ? copysignf(1e-30f, r->d[k]) : r->d[k];
+        float inv = gpu_rcp(e, var);
+        float oi = r->o[k] * inv;
+        float sc = ldexpf(1.0f, ex) * inv;
+        float base = fmaf(org, inv, -oi);
+        float tl = fmaf((float)l, sc, base), th = fmaf((float)h, sc, base);
+        tn = fmaxf(tn, inv >= 0.0f ? tl : th);
+        tf = fminf(tf, inv >= 0.0f ? th : tl);
+    }
+    return tn <= fmaf(fabsf(tf), rel, tf);
+}
+
+void orc_gpu_box_check(const rtq_bvh_node* nodes, uint32_t n_nodes, const rtq_ray* rays, const rtq_hit* hits,
+                       uint64_t n_rays, float pad, float rel, uint64_t* lost_binary, uint64_t* lost_wide) {
+    uint32_t* parent = (uint32_t*)malloc(sizeof(uint32_t) * (n_nodes ? n_nodes : 1));
+    uint32_t n_tris = (n_nodes + 1) / 2;
+    uint32_t* leaf_of = (uint32_t*)malloc(sizeof(uint32_t) * (n_tris ? n_tris : 1));
+    for (uint32_t i = 0; i < n_nodes; ++i) parent[i] = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < n_nodes; ++i) {
+        const rtq_bvh_node* nd = &nodes[i];
+        if (nd->left_child == 0 && nd->right_child == 0) {
+            if (nd->triangle_id < n_tris) leaf_of[nd->triangle_id] = i;
+        } else {
+            parent[nd->left_child] = i;
+            parent[nd->right_child] = i;
+        }
+    }
+    uint64_t lb = 0, lw = 0;
+    for (uint64_t r = 0; r < n_rays; ++r) {
+        if (hits[r].id < 0 || (uint32_t)hits[r].id >= n_tris) continue;
+        int bad_b = 0, bad_w = 0;
+        for (int var = 0; var < 3; ++var) {
+            for (uint32_t x = leaf_of[hits[r].id]; x != 0xFFFFFFFFu; x = parent[x]) {
+                if (!gpu_box_binary(&rays[r], var, &nodes[x].box, pad, hits[r].t, rel)) bad_b = 1;
+                if (parent[x] != 0xFFFFFFFFu &&
+                    !gpu_box_wide(&rays[r], var, &nodes[x].box, &nodes[parent[x]].box, pad, hits[r].t, rel))
+                    bad_w = 1;
+            }
+        }
+        lb += (uint64_t)bad_b;
+        lw += (uint64_t)bad_w;
+    }
+    free(parent);
+    free(leaf_of);
+    *lost_binary = lb;
+    *lost_wide = lw;
 }
 
 /* ------------------------------------------------------------------------ */

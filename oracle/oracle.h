@@ -25,7 +25,12 @@
  *                                    every triangle in exactly one leaf)
  *   orc_refmt / orc_watertight       closed-form hits (axis-aligned triangles),
  *                                    FP64 intersection, edge/vertex sharing
- *   orc_trace_ref / orc_trace_bvh    equality with orc_trace_brute
+ *   orc_trace_ref / orc_trace_bvh    equality with orc_trace_brute (orc_trace_bvh
+ *                                    also for ray origins 10^2-10^7 scene
+ *                                    extents away)
+ *   orc_gpu_box_check                its count is zero for hits it must keep
+ *                                    and positive for the unwidened form on
+ *                                    far-away origins (a planted mistake)
  *   orc_raygen_*                     closed forms (centre pixel, unit length,
  *                                    cosine-weighted moments)
  *   orc_mlp_grad                     central finite differences of
@@ -146,6 +151,15 @@ void orc_trace_bvh(const rtq_bvh_node* nodes, const float* verts9, const rtq_ray
                    uint64_t n_rays, int leaf, int mode, float pad, rtq_hit* hits, int n_threads,
                    uint64_t* visits, uint64_t* tests);
 
+/* The GPU kernels' box tests restated (trace.hip visit(): FMA plane distances
+ * from the approximate reciprocal, boxes padded by `pad`, far distance widened
+ * by |tf| * rel; the 8-wide form quantised outward in the parent box's frame).
+ * For each ray with a hit, every ancestor box of the hit triangle's leaf in the
+ * pre-order array is tested with tbest = the hit's t and the reciprocal rounded
+ * to nearest and one ulp either way; *lost_binary / *lost_wide count the rays
+ * with at least one ancestor box that would be skipped. */
+void orc_gpu_box_check(const rtq_bvh_node* nodes, uint32_t n_nodes, const rtq_ray* rays, const rtq_hit* hits,
+                       uint64_t n_rays, float pad, float rel, uint64_t* lost_binary, uint64_t* lost_wide);
 /* Counter-based RNG: SplitMix64 of (seed, stream, index); u01 = top 24 bits / 2^24. */
 uint64_t orc_rng_hash(uint64_t seed, uint64_t stream, uint64_t index);
 float orc_u01f(uint64_t seed, uint64_t stream, uint64_t index);

@@ -16,8 +16,18 @@
 //    through them, and an LDS read replaces an L2 round trip on the dependent
 //    chain of node fetches;
 //  - conservative slab test: boxes are stored widened outward by a pad of
-//    2^-18 of the scene extent, and the interval is [tmin, t_best] inclusive so
-//    a later, lower-id triangle at the same t is still reached;
+//    2^-18 of the scene's largest coordinate (errors that scale with the scene:
+//    rounding of box coordinates, of vertex translation by the origin, and the
+//    leaf test's own rounding of t and of its edge functions), and the far
+//    distance of every box is widened by |t_far| * 2^-19 before the compare
+//    (errors that scale with the distance: the approximate reciprocal, the
+//    rounding of o * inv and of the FMA, and the leaf test's translation of
+//    far-away vertices); the interval is [tmin, t_best] inclusive so a later,
+//    lower-id triangle at the same t is still reached.  The slack covers the
+//    watertight test for any ray origin; the RefMT test's barycentrics carry
+//    an error that grows with |o - p0| / cos(theta) across the ray, which no
+//    box widening bounds, so RefMT rays keep this guarantee only while that
+//    error stays below the pad;
 //  - leaf test: watertight FP32 (Woop, Benthin & Wald, JCGT 2013) with the FP64
 //    re-evaluation of edge functions that are exactly zero, or the reference's
 //    Moller-Trumbore (RefMT); closest hit keeps the smallest (t, id).
@@ -55,6 +65,12 @@ constexpr uint32_t kCounterWords = kArrive + kPartStride;
 #ifndef RTQ_STACK_LDS_WIDE
 #define RTQ_STACK_LDS_WIDE 16
 #endif
+// Relative widening of a box's far distance (tn <= tf + |tf| * kSlabRel):
+// 32 units of 2^-24, twice the sum of the distance-proportional errors of the
+// box test (reciprocal 2, o * inv 1, FMA 1, 8-wide node base 1) and of the
+// watertight test (vertex translation 1, t = T / det about 6), plus the
+// rounding of the widening itself.
+constexpr float kSlabRel = 0x1p-19f;
 constexpr uint32_t kStackLdsBinary = RTQ_STACK_LDS;
 constexpr uint32_t kStackLdsWide = RTQ_STACK_LDS_WIDE;
 // Optional VGPR budget of the trace kernels (occupancy is 512 / budget waves
@@ -374,7 +390,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 const float tfz = __builtin_fmaf((float)(((w ? fz1 : fz0) >> sh) & 0xFFu), scz, bz);
                 const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
                 const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest));
-                const bool hit = (tn <= tf) && ((valid >> j) & 1u);
+                const bool hit = (tn <= __builtin_fmaf(fabsf(tf), kSlabRel, tf)) && ((valid >> j) & 1u);
                 // order key: entry distance (non-negative floats order as uints), slot in the low bits
                 key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
             }
@@ -423,7 +439,8 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             const float tf0 = fminf(fminf(fmaxf(ax0, bx0), fmaxf(ay0, by0)), fminf(fmaxf(az0, bz0), tbest));
             const float tn1 = fmaxf(fmaxf(fminf(ax1, bx1), fminf(ay1, by1)), fmaxf(fminf(az1, bz1), tmin));
             const float tf1 = fminf(fminf(fmaxf(ax1, bx1), fmaxf(ay1, by1)), fminf(fmaxf(az1, bz1), tbest));
-            const bool h0 = tn0 <= tf0, h1 = tn1 <= tf1;
+            const bool h0 = tn0 <= __builtin_fmaf(fabsf(tf0), kSlabRel, tf0);
+            const bool h1 = tn1 <= __builtin_fmaf(fabsf(tf1), kSlabRel, tf1);
             const uint32_t r0 = __float_as_uint(n3.x), r1 = __float_as_uint(n3.y);
             if (h0 && h1) {
                 bool first0;

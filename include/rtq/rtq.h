@@ -231,6 +231,23 @@ rtq_status rtq_trace_stats(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_
                            uint64_t n_rays, uint32_t flags, rtq_hit* hits_dev,
                            uint32_t* node_visits_dev, uint32_t* tri_tests_dev, void* stream);
 
+/* rtq_trace without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same rays,
+ * flags, layouts and hit records as rtq_trace.  The work counters and the
+ * traversal stacks' spill live in a fixed device array of
+ * RTQ_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the
+ * process ends (rtq_create looks up its address once per device).  The
+ * persistent grid takes as many workgroups as that array holds stacks for, so
+ * very deep trees trace with fewer workgroups; RTQ_ECAPACITY when not even one
+ * workgroup's stacks fit.  Every call clears the counters first.  All calls on
+ * one GPU share the array: calls that overlap on one GPU (any context, any
+ * stream) race, and the caller must serialise them.  Asynchronous on `stream`;
+ * device buffers. */
+#define RTQ_NOALLOC_SCRATCH_BYTES (64u << 20)
+rtq_status rtq_trace_noalloc(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays,
+                             uint32_t flags, rtq_hit* hits_dev, void* stream);
+
 /* Synchronous convenience form on host arrays (copies in, traces, copies out). */
 rtq_status rtq_trace_host(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays, uint64_t n_rays,
                           uint32_t flags, rtq_hit* hits);

@@ -299,7 +299,7 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        scratch_alloc(ctx, &ctx->scratch) != RTQ_OK) {
+        scratch_alloc(ctx, &ctx->scratch) != RTQ_OK || noalloc_setup(ctx) != RTQ_OK) {
         scratch_free(&ctx->scratch);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -618,7 +618,8 @@ rtq_status rtq_scan_exclusive_u32(rtq_ctx* ctx, const uint32_t* in_dev, uint32_t
 }
 
 static rtq_status trace_common(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays, uint64_t n, uint32_t flags,
-                               rtq_hit* hits, uint32_t* sn, uint32_t* stt, bool stats, void* stream) {
+                               rtq_hit* hits, uint32_t* sn, uint32_t* stt, bool stats, void* stream,
+                               bool noalloc = false) {
     if (!ctx) return RTQ_EINVAL;
     Scene* s = find_scene(ctx, scene_id);
     if (!s) return fail(ctx, RTQ_EINVAL, "trace: unknown scene %u", scene_id);
@@ -628,12 +629,18 @@ static rtq_status trace_common(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* r
     if (n == 0) return RTQ_OK;
     if (!rays || !hits) return fail(ctx, RTQ_EINVAL, "trace: null ray or hit buffer");
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    if (noalloc) return trace_noalloc(ctx, s, rays, n, flags, hits, pick_stream(ctx, stream));
     return trace(ctx, s, rays, n, flags, hits, sn, stt, stats, pick_stream(ctx, stream));
 }
 
 rtq_status rtq_trace(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays, uint32_t flags,
                      rtq_hit* hits_dev, void* stream) {
     return trace_common(ctx, scene_id, rays_dev, n_rays, flags, hits_dev, nullptr, nullptr, false, stream);
+}
+
+rtq_status rtq_trace_noalloc(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays,
+                             uint32_t flags, rtq_hit* hits_dev, void* stream) {
+    return trace_common(ctx, scene_id, rays_dev, n_rays, flags, hits_dev, nullptr, nullptr, false, stream, true);
 }
 
 rtq_status rtq_trace_stats(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_dev, uint64_t n_rays, uint32_t flags,

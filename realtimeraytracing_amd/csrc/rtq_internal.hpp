@@ -106,6 +106,9 @@ struct TraceScratch {
     uint32_t* counter = nullptr;
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
+    // fixed: the spill buffer cannot grow (rtq_trace_noalloc); a launch takes
+    // fewer workgroups instead, so every spill column fits in spill_words
+    bool fixed = false;
 };
 }  // namespace rtq
 
@@ -238,6 +241,11 @@ rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, 
 rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
                        bool size_only, hipStream_t st);
 rtq_status scratch_alloc(rtq_ctx* ctx, TraceScratch* sc);
+// rtq_trace_noalloc (trace.hip): per-device address of the fixed scratch
+// array, looked up once per device at context creation (device current).
+rtq_status noalloc_setup(rtq_ctx* ctx);
+rtq_status trace_noalloc(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags,
+                         rtq_hit* hits, hipStream_t st);
 void scratch_free(TraceScratch* sc);
 // Compressed 8-wide nodes from the cluster arrays (fl: first leaf rank per cluster).
 rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st);

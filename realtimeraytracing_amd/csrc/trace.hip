@@ -34,6 +34,7 @@
 // Float expressions of the leaf tests are evaluated in the order of the CPU
 // restatement (built with -ffp-contract=off), so t/u/v are bit-identical.
 #include <algorithm>
+#include <mutex>
 
 #include "rtq_internal.hpp"
 
@@ -742,10 +743,22 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
     if (blocks == 0) blocks = 1;
-    const uint64_t threads = blocks * kTraceThreads;
     constexpr uint32_t kStackLds = LAYOUT == kWide ? kStackLdsWide : kStackLdsBinary;
-    const uint64_t spill_words = depth_entries > kStackLds ? (depth_entries - kStackLds) * threads : 1;
+    const uint64_t column = depth_entries > kStackLds ? depth_entries - kStackLds : 0;  // spill words per thread
+    if (sc.fixed && column > 0) {
+        // a spill buffer that cannot grow: as many workgroups as it holds
+        const uint64_t fit = sc.spill_words / (column * kTraceThreads);
+        if (fit == 0)
+            return fail(ctx, RTQ_ECAPACITY,
+                        "trace: a workgroup's traversal stacks (%llu spill entries per ray) exceed the fixed "
+                        "scratch of %llu bytes", (unsigned long long)column,
+                        (unsigned long long)(sc.spill_words * sizeof(uint32_t)));
+        blocks = std::min(blocks, fit);
+    }
+    const uint64_t threads = blocks * kTraceThreads;
+    const uint64_t spill_words = column ? column * threads : 1;
     if (spill_words > sc.spill_words) {
+        if (sc.fixed) return fail(ctx, RTQ_EINTERNAL, "trace: fixed scratch sizing");
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         RTQ_HIP(ctx, hipStreamIsCapturing(st, &cs));
         if (cs != hipStreamCaptureStatusNone)
@@ -784,14 +797,17 @@ uint64_t stack_bound(const Scene* s, bool wide) {
 }
 
 template <int LEAF, int MODE, bool STATS>
-rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, int layout, hipStream_t st) {
+rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, int layout, hipStream_t st,
+                         TraceScratch* fixed = nullptr) {
     const ItemTable<false> none{};
     const bool wide = layout == kWide;
     a.nodes = wide ? s->wide : s->nodes;
     a.top = wide ? nullptr : s->top;
     a.n_top = wide ? 0u : s->n_top;
-    TraceScratch& sc = ctx->scratch;
-    const uint64_t depth = stack_bound(s, wide);
+    TraceScratch& sc = fixed ? *fixed : ctx->scratch;
+    // a fixed scratch is sized per call, so it can take the current 8-wide
+    // tree's own bound (at most 7 pushes per level) instead of the binary one
+    const uint64_t depth = (fixed && wide) ? 7ull * s->wide_depth + 2 : stack_bound(s, wide);
     if (layout == kWide) return launch<LEAF, MODE, STATS, kWide, 0>(ctx, sc, depth, a, none, n, false, st);
     if (layout == kOrdered) return launch<LEAF, MODE, STATS, kOrdered, 0>(ctx, sc, depth, a, none, n, false, st);
     return launch<LEAF, MODE, STATS, kBinary, 0>(ctx, sc, depth, a, none, n, false, st);
@@ -831,6 +847,75 @@ void scratch_free(TraceScratch* sc) {
     if (sc->counter) (void)hipFree(sc->counter);
     if (sc->spill) (void)hipFree(sc->spill);
     *sc = TraceScratch{};
+}
+
+// ---- allocation-free trace ------------------------------------------------
+namespace {
+
+// Scratch of rtq_trace_noalloc: the work counters (kCounterWords words), then
+// the traversal stacks' spill columns.  A file-scope device array: each GPU
+// the library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in, never device memory: kernels and
+// copies take the per-device address from noalloc_setup instead.
+constexpr size_t kNoAllocWords = RTQ_NOALLOC_SCRATCH_BYTES / sizeof(uint32_t);
+constexpr size_t kNoAllocSpillOffset = (kCounterWords + 63) & ~size_t(63);
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+uint32_t* g_noalloc_ptr[kMaxDevices] = {};
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+}  // namespace
+
+rtq_status noalloc_setup(rtq_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return fail(ctx, RTQ_EINVAL, "device %d: at most %d devices", ctx->device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[ctx->device]) return RTQ_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) return hip_fail(ctx, e, "hipGetSymbolAddress(scratch)");
+    g_noalloc_ptr[ctx->device] = static_cast<uint32_t*>(p);
+    return RTQ_OK;
+}
+
+rtq_status trace_noalloc(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags,
+                         rtq_hit* hits, hipStream_t st) {
+    if (n == 0) return RTQ_OK;
+    if (n > 0x80000000ull) return fail(ctx, RTQ_ECAPACITY, "trace: n_rays must be <= 2^31");
+    uint32_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_noalloc_ptr[ctx->device] : nullptr;
+    }
+    if (!base) return fail(ctx, RTQ_EINTERNAL, "trace_noalloc: scratch address not set up for device %d", ctx->device);
+    const int layout = pick_layout(ctx, s, flags);
+    if (layout < 0) return RTQ_ENOTBUILT;
+    TraceScratch sc;
+    sc.counter = base;
+    sc.spill = base + kNoAllocSpillOffset;
+    sc.spill_words = kNoAllocWords - kNoAllocSpillOffset;
+    sc.fixed = true;
+    // the counters start at zero (a launch's last block also resets them)
+    hipLaunchKernelGGL(k_zero_u32, dim3((kCounterWords + 255) / 256), dim3(256), 0, st, sc.counter,
+                       (uint32_t)kCounterWords);
+    RTQ_HIP(ctx, hipGetLastError());
+    TraceArgs args{};
+    args.tris = s->tris;
+    args.rays = rays;
+    args.hits = hits;
+    args.n = (uint32_t)n;
+    const bool any = flags & RTQ_TRACE_ANY;
+    const uint32_t nn = (uint32_t)n;
+    if (!(flags & RTQ_TRACE_LEAF_REFMT))
+        return any ? launch_single<0, 1, false>(ctx, s, args, nn, layout, st, &sc)
+                   : launch_single<0, 0, false>(ctx, s, args, nn, layout, st, &sc);
+    return any ? launch_single<1, 1, false>(ctx, s, args, nn, layout, st, &sc)
+               : launch_single<1, 0, false>(ctx, s, args, nn, layout, st, &sc);
 }
 
 rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, uint32_t flags, rtq_hit* hits,

@@ -57,7 +57,7 @@ EXPORTED = [
     "rtq_order_model_create", "rtq_order_model_destroy", "rtq_order_model_log", "rtq_order_model_clear_samples",
     "rtq_order_model_train", "rtq_order_model_apply", "rtq_order_model_save", "rtq_order_model_load",
     "rtq_order_model_info", "rtq_shade", "rtq_bvh_overlay", "rtq_obj_load", "rtq_obj_parse",
-    "rtq_scene_update", "rtq_bvh_refit",
+    "rtq_scene_update", "rtq_bvh_refit", "rtq_trace_noalloc",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -142,6 +142,7 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_sort_pairs": (st, [vp, vp, vp, u64, u32, vp]),
         "rtq_scan_exclusive_u32": (st, [vp, vp, vp, u64, vp]),
         "rtq_trace": (st, [vp, u32, vp, u64, u32, vp, vp]),
+        "rtq_trace_noalloc": (st, [vp, u32, vp, u64, u32, vp, vp]),
         "rtq_trace_stats": (st, [vp, u32, vp, u64, u32, vp, vp, vp, vp]),
         "rtq_trace_host": (st, [vp, u32, vp, u64, u32, vp]),
         "rtq_raygen_pinhole": (st, [vp, ctypes.POINTER(Pinhole), vp, vp]),
@@ -475,6 +476,12 @@ class Scene:
 
     def trace(self, rays_ptr: int, n: int, hits_ptr: int, flags: int = 0, stream: int | None = None) -> None:
         self.ctx.check(self.ctx.lib.rtq_trace(self.ctx.h, self.id, rays_ptr, n, flags, hits_ptr, stream))
+
+    def trace_noalloc(self, rays_ptr: int, n: int, hits_ptr: int, flags: int = 0,
+                      stream: int | None = None) -> None:
+        """rtq_trace_noalloc: the same trace with no device allocation (fixed
+        per-GPU scratch; overlapping calls on one GPU race)."""
+        self.ctx.check(self.ctx.lib.rtq_trace_noalloc(self.ctx.h, self.id, rays_ptr, n, flags, hits_ptr, stream))
 
     def trace_stats(self, rays_ptr: int, n: int, hits_ptr: int, nodes_ptr: int | None, tris_ptr: int | None,
                     flags: int = 0, stream: int | None = None) -> None:

@@ -36,10 +36,11 @@ compressed nodes, learned order (two launches) and learned order batched in
 one graph-replayed launch, with the same exactness check.
 
 Prints one JSON line with the metric, a roofline object for the dominant
-trace kernel of the best form (algorithmic bytes per launch from the
-traversal counters over its HIP-event duration) and the CPU baseline (the
+trace kernel of the best form (HBM bytes per launch from the committed,
+calibrated rocprofv3 counters over the launch's HIP-event duration, with the
+algorithmic bytes and the L2 rate beside them) and the CPU baseline (the
 reference traversal restated in C, run on this host's cores over the same
-rays).
+rays; 1 warm-up, median of 5).
 """
 from __future__ import annotations
 
@@ -56,9 +57,100 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# L2 peak: 2 KiB per clock per XCD (16 channels x 128 B), 8 XCDs, 2.4 GHz
+L2_PEAK_GBS = 2048 * 8 * 2.4
 TRI_BYTES = 48       # leaf triangle record
 RAY_BYTES = 32
 HIT_BYTES = 16
+BINARY_NODE_BYTES = 64   # one binary node per visit
+WIDE_NODE_BYTES = 96     # an 8-wide visit reads 96 B of its 128-B slot
+# sources the traversal kernels' memory traffic depends on; the committed
+# traffic summary records their digest and is ignored when it differs
+KERNEL_SOURCES = ("trace.hip", "wide.hip", "build.hip", "batch.hip", "rtq_internal.hpp")
+
+
+def kernel_label(rocprof_name: str):
+    """bench's name of a trace kernel from its rocprof name (template arguments
+    leaf test, mode, counters, layout, batch[, occupancy]), or None."""
+    import re
+    m = re.search(r"k_trace(?:_occ)?<(\d+), (\d+), (false|true), (\d+), (\d+)", rocprof_name)
+    if not m or m.group(3) == "true":
+        return None
+    leaf = "watertight" if m.group(1) == "0" else "refmt"
+    mode = {"0": "closest", "1": "any", "2": "per-item"}[m.group(2)]
+    layout = {"0": "binary", "1": "wide", "2": "learned"}[m.group(4)]
+    return f"k_trace<{leaf},{mode},{layout}" + (",batch>" if m.group(5) != "0" else ">")
+
+
+def summarize_profile(out_path: str, dirs: list[str], commands: list[str], last: int = 5) -> str:
+    """Writes the traffic summary read by read_profile from rocprofv3 --pmc
+    CSVs (FETCH_SIZE, WRITE_SIZE, TCC_READ_SECTORS_sum + TCC_WRITE_SECTORS_sum
+    passes of the same runs, each ending with a 512 MiB device copy): per
+    trace kernel the median over its last `last` dispatches in the run that
+    dispatches it most, each counter scaled by what it reads for the copy
+    (512 MiB read, 512 MiB written)."""
+    import collections
+    import csv
+    copy_bytes = 512 << 20
+    # counter -> label -> (dispatches in the run, median of its last ones); the
+    # run that dispatches a kernel most holds its timed launches (every run
+    # also traces its primary rays once with the binary closest-hit kernel)
+    vals = collections.defaultdict(dict)
+    scale = {}
+    for d in dirs:
+        per = collections.defaultdict(lambda: collections.defaultdict(float))   # dispatch -> counter -> sum
+        names = {}
+        for f in Path(d).glob("*counter_collection.csv"):
+            for r in csv.DictReader(open(f)):
+                key = int(r["Dispatch_Id"])
+                per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+                names[key] = r["Kernel_Name"]
+        by_label = collections.defaultdict(list)
+        copies = []
+        for key in sorted(per):
+            if "copyBuffer" in names[key]:
+                copies.append(key)
+            lab = kernel_label(names[key])
+            if lab:
+                by_label[lab].append(key)
+        for counter in {c for k in per for c in per[k]}:
+            if copies:
+                best = max(per[k].get(counter, 0.0) for k in copies[-3:])
+                if counter in ("FETCH_SIZE", "WRITE_SIZE"):
+                    scale[counter] = copy_bytes / (best * 1024.0) if best else None
+                else:
+                    scale[counter] = (copy_bytes / 32.0) / best if best else None
+            for lab, keys in by_label.items():
+                cand = (len(keys), float(np.median([per[k].get(counter, 0.0) for k in keys[-last:]])))
+                if cand[0] > vals[counter].get(lab, (0, 0.0))[0]:
+                    vals[counter][lab] = cand
+    lines = ["# HBM and L2 bytes per launch of the terrain trace kernels (rocprofv3 --pmc, calibrated).",
+             "# A 512 MiB device copy at the end of each run gives the scale of every counter: bytes =",
+             "# FETCH_SIZE x 1024 x s_F + WRITE_SIZE x 1024 x s_W (HBM) and 32 B x (TCC_READ_SECTORS x s_R",
+             "# + TCC_WRITE_SECTORS x s_W2) (L2); medians over each kernel's last %d dispatches in the run" % last,
+             "# that dispatches it most (its timed launches; batched forms run last in their pass).",
+             "# scales: " + ", ".join(f"{c} {v:.4f}" for c, v in sorted(scale.items()) if v),
+             "# commands (repository root $R, from /tmp with TMPDIR=/tmp):"]
+    lines += [f"#   {c}" for c in commands]
+    lines.append(f"kernel_sources_sha256: {kernel_digest()}")
+    labels = sorted({lab for c in vals for lab in vals[c]})
+    def get(counter, lab):
+        v = vals[counter].get(lab)
+        return v[1] if v else None
+
+    for lab in labels:
+        f_, w_ = get("FETCH_SIZE", lab), get("WRITE_SIZE", lab)
+        if f_ is not None and w_ is not None and scale.get("FETCH_SIZE") and scale.get("WRITE_SIZE"):
+            hbm = f_ * 1024 * scale["FETCH_SIZE"] + w_ * 1024 * scale["WRITE_SIZE"]
+            lines.append(f"hbm_bytes_per_launch {lab}: {hbm:.0f}")
+        r_, w2 = get("TCC_READ_SECTORS_sum", lab), get("TCC_WRITE_SECTORS_sum", lab)
+        if r_ is not None and w2 is not None and scale.get("TCC_READ_SECTORS_sum") and \
+                scale.get("TCC_WRITE_SECTORS_sum"):
+            l2 = 32.0 * (r_ * scale["TCC_READ_SECTORS_sum"] + w2 * scale["TCC_WRITE_SECTORS_sum"])
+            lines.append(f"l2_bytes_per_launch {lab}: {l2:.0f}")
+    text = "\n".join(lines) + "\n"
+    Path(out_path).write_text(text)
+    return text
 
 
 def parse():
@@ -81,22 +173,69 @@ def parse():
     ap.add_argument("--no-cpu-reference-case", action="store_true",
                     help="skip the 65,536-triangle soup against the CPU reference traversal")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
+    ap.add_argument("--cpu-repeats", type=int, default=5, help="CPU baseline: timed runs after one warm-up (median)")
+    ap.add_argument("--summarize-profile", nargs="+", metavar=("OUT", "PMC_DIR"),
+                    help="write the traffic summary OUT from rocprofv3 --pmc output directories and exit")
+    ap.add_argument("--profile-commands", nargs="*", default=[], help="commands recorded in the summary")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
                          "known traffic to scale rocprof FETCH_SIZE / WRITE_SIZE readings")
     return ap.parse_args()
 
 
-def read_traffic(path: str, kernel: str):
-    """HBM bytes per launch of a kernel from the committed rocprof counter
-    summary (lines `traffic_bytes_per_launch <kernel>: <value>`), else None."""
+def kernel_digest() -> str:
+    """SHA-256 (first 16 hex digits) of the traversal kernels' sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update((ROOT / "realtimeraytracing_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def read_profile(path: str) -> dict:
+    """Per-kernel bytes per launch from the committed rocprof counter summary:
+    `hbm_bytes_per_launch <kernel>: v` (calibrated FETCH_SIZE + WRITE_SIZE) and
+    `l2_bytes_per_launch <kernel>: v` (TCC read + write sectors x 32 B), with
+    the kernel-source digest they were measured at (`kernel_sources_sha256: d`).
+    `fresh` is False when the digest differs from the current sources."""
+    out = {"hbm": {}, "l2": {}, "digest": None, "fresh": False, "path": path}
     p = Path(path)
     if not p.exists():
-        return None
+        return out
     for line in p.read_text().splitlines():
-        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-            return float(line.split(":", 1)[1])
-    return None
+        for key, field in (("hbm_bytes_per_launch ", "hbm"), ("l2_bytes_per_launch ", "l2")):
+            if line.startswith(key):
+                k, v = line[len(key):].rsplit(":", 1)
+                out[field][k.strip()] = float(v)
+        if line.startswith("kernel_sources_sha256:"):
+            out["digest"] = line.split(":", 1)[1].strip()
+    out["fresh"] = out["digest"] == kernel_digest()
+    return out
+
+
+def roofline_of(prof: dict, kernel: str, launch_ms: float, alg_bytes: int, node_bytes: int) -> dict:
+    """HBM roofline of one launch: calibrated DRAM bytes over its HIP-event
+    time against the 8 TB/s peak; beside it the algorithmic bytes (ray + hit +
+    node and triangle fetches, most of them served by L2 and the Infinity
+    Cache) and the L2 request bytes against the L2 peak."""
+    s = launch_ms * 1e-3
+    hbm = prof["hbm"].get(kernel) if prof["fresh"] else None
+    l2 = prof["l2"].get(kernel) if prof["fresh"] else None
+    achieved = hbm / s / 1e9 if hbm else None
+    l2_gbs = l2 / s / 1e9 if l2 else None
+    src = (f"{Path(prof['path']).name}: rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated) and TCC sectors, kernel "
+           f"sources {prof['digest']}") if prof["fresh"] else \
+        (f"stale: {Path(prof['path']).name} was measured at kernel sources {prof['digest']}, "
+         f"current {kernel_digest()}")
+    return {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": hbm, "kernel": kernel, "launch_ms": round(launch_ms, 4), "traffic_source": src,
+            "algorithmic": {"bytes_per_launch": alg_bytes, "GB_s": round(alg_bytes / s / 1e9, 1),
+                            "bytes_per_ray": f"32 ray + 16 hit + {node_bytes} x node visits + 48 x triangle tests",
+                            "note": "node and triangle fetches mostly hit L2 and the 256 MB Infinity Cache, so "
+                                    "this rate is not an HBM rate"},
+            "l2": {"bytes_per_launch": l2, "GB_s": round(l2_gbs, 1) if l2_gbs else None, "peak_GB_s": L2_PEAK_GBS,
+                   "frac": round(l2_gbs / L2_PEAK_GBS, 4) if l2_gbs else None}}
 
 
 class Workload:
@@ -205,6 +344,9 @@ def t_error_stats(tris, rays, hits, n_sample=20_000, seed=0):
 
 def main() -> int:
     args = parse()
+    if args.summarize_profile:
+        print(summarize_profile(args.summarize_profile[0], args.summarize_profile[1:], args.profile_commands))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU only; the trace path "
                                    "would shard by ray ranges over a BVH replicated on every GPU"}))
@@ -266,7 +408,9 @@ def main() -> int:
     setup_s = time.time() - t0
 
     default_params = ctx.launch_params()
-    counters_by_layout = {"binary": wl.counters(RTQ_TRACE_BINARY, 64), "wide": wl.counters(0, 128)}
+    counters_by_layout = {"binary": wl.counters(RTQ_TRACE_BINARY, BINARY_NODE_BYTES),
+                          "wide": wl.counters(0, WIDE_NODE_BYTES)}
+    profile = read_profile(args.traffic_file)
     reference = None
     results = {}
     learned_setup = None
@@ -305,7 +449,7 @@ def main() -> int:
             if form == "learned" and wl.sc.info().order_modes:
                 # learned decisions: the batch traces binary nodes with the learned order
                 lname = "learned"
-                counters_by_layout["learned"] = wl.counters(RTQ_TRACE_LEARNED | RTQ_TRACE_BINARY, 64)
+                counters_by_layout["learned"] = wl.counters(RTQ_TRACE_LEARNED | RTQ_TRACE_BINARY, BINARY_NODE_BYTES)
             else:
                 # the batched form, or learned order without decisions (the plain
                 # order), on the layout the autotuner chose
@@ -324,17 +468,12 @@ def main() -> int:
         got = wl.hits()
         if reference is None:
             reference = got
-        achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-        node_bytes = 128 if lname == "wide" else 64
+        node_bytes = WIDE_NODE_BYTES if lname == "wide" else BINARY_NODE_BYTES
         results[form] = {
             "Mrays_s": wl.live_rays / (ms_per_step * 1e-3) / 1e6,
             "ms_per_step": ms_per_step, "diffuse_ms": diff_ms, "shadow_ms": shad_ms,
             "layout": lname, "tuned": tuned, "counters": counters, "exact": same_results(reference, got),
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.traffic_file, kernel),
-                         "kernel": kernel, "launch_ms": round(dom_ms, 4),
-                         "algorithmic_bytes_per_launch": dom_bytes,
-                         "bytes_per_ray": "32 ray + 16 hit + %d x node visits + 48 x triangle tests" % node_bytes},
+            "roofline": roofline_of(profile, kernel, dom_ms, dom_bytes, node_bytes),
         }
     ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
                               default_params.spec_slack, default_params.leaf_slack)
@@ -354,17 +493,23 @@ def main() -> int:
         shad = devmem.to_host(wl.shad, RAY_DTYPE)
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = int(os.environ.get("OMP_NUM_THREADS", cores))
-        t1 = time.perf_counter()
-        oracle.trace_ref(nodes, v9, diff, leaf=oracle.LEAF_REFMT, threads=cores)
-        oracle.trace_ref(nodes, v9, shad, leaf=oracle.LEAF_REFMT, threads=cores)
-        cpu_s = time.perf_counter() - t1
-        t2 = time.perf_counter()
-        oracle.trace_bvh(nodes, v9, diff, threads=cores)
-        oracle.trace_bvh(nodes, v9, shad, mode=oracle.MODE_ANY, threads=cores)
-        cpu2_s = time.perf_counter() - t2
+        def median_s(fn):
+            fn()                                  # warm-up
+            ts = []
+            for _ in range(max(1, args.cpu_repeats)):
+                t1 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t1)
+            return float(np.median(ts)), ts
+
+        cpu_s, cpu_ts = median_s(lambda: (oracle.trace_ref(nodes, v9, diff, leaf=oracle.LEAF_REFMT, threads=cores),
+                                          oracle.trace_ref(nodes, v9, shad, leaf=oracle.LEAF_REFMT, threads=cores)))
+        cpu2_s, _ = median_s(lambda: (oracle.trace_bvh(nodes, v9, diff, threads=cores),
+                                      oracle.trace_bvh(nodes, v9, shad, mode=oracle.MODE_ANY, threads=cores)))
         cpu = {"value": round(wl.live_rays / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
                "sample": f"all {wl.n_diff} diffuse + {wl.n_shad} shadow rays of one step, reference traversal "
-                         f"semantics (RefMT, no pruning), {cpu_s:.2f} s",
+                         f"semantics (RefMT, no pruning); 1 warm-up, median of {len(cpu_ts)} runs "
+                         f"({min(cpu_ts):.2f}-{max(cpu_ts):.2f} s)",
                "watertight_pruned_Mrays_s": round(wl.live_rays / cpu2_s / 1e6, 3)}
 
     # caller side of the path: shading the primary hits into an RGBA32F image
@@ -465,6 +610,22 @@ def main() -> int:
                  "fused_ms": round(times["fused"], 3), "split_ms": round(times["split"], 3),
                  "fused_vs_split": round(times["split"] / times["fused"], 3), "ploc_passes": pf,
                  "same_clusters": bool(same)}
+        if not args.no_cpu_baseline:
+            # the reference's build restated in C (bounds, Morton codes, pair sort,
+            # PLOC with its nearest-neighbour search in parallel, pre-order flatten)
+            import oracle
+            cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+            cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+            oracle.build_ref_nodes(tris, models)          # warm-up
+            cts = []
+            for _ in range(3):
+                t_b = time.perf_counter()
+                _, cpasses = oracle.build_ref_nodes(tris, models)
+                cts.append((time.perf_counter() - t_b) * 1e3)
+            build["cpu_reference_build_ms"] = round(float(np.median(cts)), 1)
+            build["cpu_reference_build"] = (f"orc_build_ref_nodes, {cores} threads, 1 warm-up + median of 3, "
+                                            f"{cpasses} PLOC passes (same as the GPU: {cpasses == pf})")
+            build["gpu_vs_cpu_build"] = round(build["cpu_reference_build_ms"] / times["fused"], 1)
 
     # scene edit: every terrain vertex moved (heights rippled by up to 0.02),
     # then a refit of the kept tree against a full rebuild
@@ -533,7 +694,7 @@ def main() -> int:
             got = owl.hits()
             if oref is None:
                 oref = got
-            c = owl.counters(flags, 128 if form == "compressed" else 64)
+            c = owl.counters(flags, WIDE_NODE_BYTES if form == "compressed" else BINARY_NODE_BYTES)
             split = form != "learned_batched"
             oforms[form] = {
                 "Mrays_s": round(owl.live_rays / (ms_per_step * 1e-3) / 1e6, 2),

@@ -229,7 +229,9 @@ int orc_ploc_build(const rtq_triangle* tris, uint32_t n, const rtq_mesh_model* m
     uint32_t iter = n, n_total = n;
     int passes = 0;
     while (iter > 1) {
-        /* plocNearestNeighborSearch, bvh.cpp:193-210 */
+        /* plocNearestNeighborSearch, bvh.cpp:193-210, positions in parallel as
+         * the reference's OpenMP loop runs them (each result is independent) */
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < iter; ++i) {
             float min_dist = INFINITY;
             const rtq_aabb* bi = &boxes[c_in[i]];

@@ -67,6 +67,7 @@ void reset_build(Scene* s) {
     s->n_trav = 0;
     s->n_wide = 0;
     s->wide_depth = 0;
+    s->ref_ready = false;
 }
 
 void free_build(Scene* s) {
@@ -80,6 +81,7 @@ void free_build(Scene* s) {
     release(s->sorted_idx);
     release(s->sorted_codes);
     release(s->ref_nodes);
+    release(s->preorder);
     release(s->top);
     release(s->tris);
 }
@@ -523,6 +525,8 @@ rtq_status rtq_bvh_export_ref(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_node* nod
     if (!nodes) return RTQ_OK;
     if (capacity < s->n_clusters) return fail(ctx, RTQ_ECAPACITY, "bvh_export_ref: need %u nodes", s->n_clusters);
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    const rtq_status rs = ensure_ref_nodes(ctx, s, ctx->stream);
+    if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipMemcpyAsync(nodes, s->ref_nodes, sizeof(rtq_bvh_node) * s->n_clusters, hipMemcpyDeviceToHost, ctx->stream));
     RTQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RTQ_OK;

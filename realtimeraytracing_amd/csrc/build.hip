@@ -865,39 +865,45 @@ __global__ __launch_bounds__(kSmallThreads) void k_place_small(const uint2* __re
     }
 }
 
-// Reference node array (scene.cpp:189-208) + leaf-ordered traversal triangles.
-__global__ __launch_bounds__(kThreads) void k_write_ref(
-    uint32_t n_clusters, const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ ctri,
-    const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
-    const uint32_t* __restrict__ pos, const uint32_t* __restrict__ fl, const uint32_t* __restrict__ dep,
-    const float4* __restrict__ tri_world, rtq_bvh_node* __restrict__ ref, float4* __restrict__ tris,
-    uint32_t* __restrict__ max_depth) {
+// Leaf-ordered traversal triangles and the depth of the deepest leaf.
+__global__ __launch_bounds__(kThreads) void k_write_leaves(
+    uint32_t n_clusters, const uint32_t* __restrict__ ctri, const uint32_t* __restrict__ cleft,
+    const uint32_t* __restrict__ fl, const uint32_t* __restrict__ dep, const float4* __restrict__ tri_world,
+    float4* __restrict__ tris, uint32_t* __restrict__ max_depth) {
     const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
     uint32_t depth = 0;
-    if (c < n_clusters) {
-        rtq_bvh_node nd{};
-        nd.box = cbox[c];
-        nd.box.pad0_ = 0.0f;
-        nd.box.pad1_ = 0.0f;
-        const bool leaf = cleft[c] == kNone;
-        nd.triangle_id = leaf ? ctri[c] : 0u;
-        nd.left_child = leaf ? 0u : pos[cleft[c]];
-        nd.right_child = leaf ? 0u : pos[cright[c]];
-        nd.pad_ = 0u;
-        ref[pos[c]] = nd;
-        if (leaf) {
-            const uint32_t t = ctri[c];
-            const uint64_t dst = 3ull * fl[c];
-            tris[dst + 0] = tri_world[3ull * t + 0];
-            tris[dst + 1] = tri_world[3ull * t + 1];
-            tris[dst + 2] = tri_world[3ull * t + 2];
-            depth = dep[c];
-        }
+    if (c < n_clusters && cleft[c] == kNone) {
+        const uint32_t t = ctri[c];
+        const uint64_t dst = 3ull * fl[c];
+        tris[dst + 0] = tri_world[3ull * t + 0];
+        tris[dst + 1] = tri_world[3ull * t + 1];
+        tris[dst + 2] = tri_world[3ull * t + 2];
+        depth = dep[c];
     }
     // deepest leaf: one atomic per wave instead of one per leaf
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) depth = max(depth, (uint32_t)__shfl_xor(depth, d, 64));
     if ((threadIdx.x & 63) == 0 && depth > 0) atomicMax(max_depth, depth);
+}
+
+// Reference node array (scene.cpp:189-208): cluster c at its pre-order
+// position, children as positions, leaves with their triangle.
+__global__ __launch_bounds__(kThreads) void k_write_ref(
+    uint32_t n_clusters, const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ ctri,
+    const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright, const uint32_t* __restrict__ pos,
+    rtq_bvh_node* __restrict__ ref) {
+    const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= n_clusters) return;
+    rtq_bvh_node nd{};
+    nd.box = cbox[c];
+    nd.box.pad0_ = 0.0f;
+    nd.box.pad1_ = 0.0f;
+    const bool leaf = cleft[c] == kNone;
+    nd.triangle_id = leaf ? ctri[c] : 0u;
+    nd.left_child = leaf ? 0u : pos[cleft[c]];
+    nd.right_child = leaf ? 0u : pos[cright[c]];
+    nd.pad_ = 0u;
+    ref[pos[c]] = nd;
 }
 
 __device__ __forceinline__ bool is_trav_node(uint32_t c, uint32_t root, uint32_t max_leaf,
@@ -1138,9 +1144,12 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     const uint32_t n = s->n_tris;
     const uint32_t nc = s->n_clusters;
     Temps tmp(ctx);
-    uint32_t *pos = nullptr, *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr,
-             *bidx = nullptr, *ntrav = nullptr;
-    RTQ_HIP(ctx, tmp.get(&pos, nc));
+    uint32_t *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr, *bidx = nullptr, *ntrav = nullptr;
+    // pre-order positions stay with the scene: the reference node array is
+    // written from them on demand (ensure_ref_nodes)
+    if (!s->preorder) RTQ_HIP(ctx, dev_alloc(&s->preorder, nc));
+    uint32_t* pos = s->preorder;
+    s->ref_ready = false;
     RTQ_HIP(ctx, tmp.get(&fl, nc));
     RTQ_HIP(ctx, tmp.get(&dep, nc));
     RTQ_HIP(ctx, tmp.get(&maxd, 2));
@@ -1165,8 +1174,8 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
                            s->cleft, s->cright, s->ccount, pos, fl, dep);
         RTQ_HIP(ctx, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_write_ref, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->cbox, s->ctri,
-                       s->cleft, s->cright, pos, fl, dep, s->tri_world, s->ref_nodes, s->tris, maxd);
+    hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->ctri, s->cleft, fl,
+                       dep, s->tri_world, s->tris, maxd);
     RTQ_HIP(ctx, hipGetLastError());
 
     rtq_aabb h_bounds[2];
@@ -1218,6 +1227,20 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
 
 }  // namespace
 
+rtq_status ensure_ref_nodes(rtq_ctx* ctx, Scene* s, hipStream_t st) {
+    if (s->ref_ready) return RTQ_OK;
+    const uint32_t nc = s->n_clusters;
+    if (!s->built || !s->preorder || nc == 0)
+        return fail(ctx, RTQ_EINTERNAL, "reference nodes: scene has no placed tree");
+    if (!s->ref_nodes) RTQ_HIP(ctx, dev_alloc(&s->ref_nodes, nc));
+    if (!s->ref_nodes) return fail(ctx, RTQ_ENOMEM, "reference nodes: allocation failed");
+    hipLaunchKernelGGL(k_write_ref, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->cbox, s->ctri, s->cleft,
+                       s->cright, s->preorder, s->ref_nodes);
+    RTQ_HIP(ctx, hipGetLastError());
+    s->ref_ready = true;
+    return RTQ_OK;
+}
+
 rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     hipStream_t st = ctx->stream;
     const uint32_t n = s->n_tris;
@@ -1252,7 +1275,6 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     if (!s->ccount) RTQ_HIP(ctx, dev_alloc(&s->ccount, nc));
     if (!s->sorted_idx) RTQ_HIP(ctx, dev_alloc(&s->sorted_idx, n));
     if (!s->sorted_codes) RTQ_HIP(ctx, dev_alloc(&s->sorted_codes, n));
-    if (!s->ref_nodes) RTQ_HIP(ctx, dev_alloc(&s->ref_nodes, nc));
     if (!s->tris) RTQ_HIP(ctx, dev_alloc(&s->tris, 3ull * n));
 
     Temps tmp(ctx);

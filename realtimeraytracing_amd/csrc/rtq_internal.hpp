@@ -64,7 +64,9 @@ struct Scene {
     uint32_t* ccount = nullptr;         // leaves below each cluster
     uint32_t* sorted_idx = nullptr;     // N
     uint32_t* sorted_codes = nullptr;   // N
-    rtq_bvh_node* ref_nodes = nullptr;  // 2N - 1 pre-order nodes (reference layout)
+    rtq_bvh_node* ref_nodes = nullptr;  // 2N - 1 pre-order nodes (reference layout), written on demand
+    bool ref_ready = false;             // ref_nodes matches the current tree and boxes
+    uint32_t* preorder = nullptr;       // pre-order position of every cluster (2N - 1)
     float4* nodes = nullptr;            // traversal nodes, 4 float4 (64 B) each
     uint4* node_aux = nullptr;          // per traversal node: child leaf ranges [A,B) [B,C), depth
     bool ordered = false;               // learned child-order words stored in the nodes
@@ -254,6 +256,10 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s);
 // (Re)writes s->top from s->nodes: after the traversal nodes or their order
 // words change (build.hip).
 rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st);
+// Writes s->ref_nodes (the reference's pre-order BVH_NodeGPU array) from the
+// cluster arrays when the build or refit since the last call changed them;
+// queued on st (build.hip).
+rtq_status ensure_ref_nodes(rtq_ctx* ctx, Scene* s, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
 // Queued on st and not waited for; digit counts come from the context's arena.
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,

@@ -175,6 +175,8 @@ rtq_status rtq_bvh_overlay(rtq_ctx* ctx, uint32_t scene_id, const rtq_ray* rays_
         return fail(ctx, RTQ_ECAPACITY, "bvh_overlay: depth %u exceeds %u", depth, kOverlayStack - 2);
     // BVH_LINE_WIDTH / (uDepthDisplayBVH + 1.f), raytracer.glsl:222
     const float threshold = kBvhLineWidth / ((float)depth + 1.0f);
+    const rtq_status rs = ensure_ref_nodes(ctx, s, st);
+    if (rs != RTQ_OK) return rs;
     hipLaunchKernelGGL(k_bvh_overlay, dim3(shade_blocks(n)), dim3(kShadeThreads), 0, st, s->ref_nodes, rays_dev, n,
                        depth, threshold, out);
     RTQ_HIP(ctx, hipGetLastError());

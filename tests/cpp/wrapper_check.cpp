@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "rtq/bvh.hpp"
+#include "rtq/cr_bvh.hpp"
 
 #define CHECK(c)                                                         \
     do {                                                                 \
@@ -25,6 +26,38 @@
 struct Vec4 { float x, y, z, w; };
 struct TriangleGPU { Vec4 p0, p1, p2; alignas(16) uint32_t model_id; };
 struct MeshModelGPU { float m[16]; uint32_t material_id; };
+// what the reference's triangle.hpp / mesh.hpp give a caller
+namespace cr {
+using TriangleGPU = ::TriangleGPU;
+using MeshModelGPU = ::MeshModelGPU;
+}
+
+// The reference Scene's use of cr::BVH (scene.cpp:148, 189-208), written
+// against the drop-in header: build in the constructor, then a recursive
+// pre-order walk over _InternalStruct from the root cluster 2N - 2.
+struct ReferenceStyleScene {
+    uint32_t _NbTriangles = 0;
+    cr::BVH_Ptr _BVH;
+
+    void walk(std::vector<cr::BVH_NodeGPU>& out, cr::BVH_Ptr bvh, uint32_t id) const {
+        cr::BVH_NodeGPU node = bvh->_InternalStruct._Clusters[id].value();
+        const uint32_t at = static_cast<uint32_t>(out.size());
+        out.push_back(node);
+        if (bvh->_InternalStruct._IsLeaf[id]) return;
+        out[at]._LeftChild = static_cast<uint32_t>(out.size());
+        walk(out, bvh, bvh->_InternalStruct._LeftChild[id].value());
+        out[at]._RightChild = static_cast<uint32_t>(out.size());
+        walk(out, bvh, bvh->_InternalStruct._RightChild[id].value());
+    }
+    std::vector<cr::BVH_NodeGPU> nodesForGpu(cr::BVH_Ptr bvh) const {
+        std::vector<cr::BVH_NodeGPU> out;
+        walk(out, bvh, 2 * _NbTriangles - 2);
+        return out;
+    }
+    void bind(const std::vector<cr::TriangleGPU>& triangleGPU, const std::vector<cr::MeshModelGPU>& modelsGPU) {
+        _BVH = cr::BVH_Ptr(new cr::BVH(_NbTriangles, triangleGPU, modelsGPU));
+    }
+};
 
 int main() {
     // a 32 x 32 flat grid at y = 0 (2 x 31^2 triangles)
@@ -41,7 +74,29 @@ int main() {
     std::vector<MeshModelGPU> models{identity};
     const uint32_t n = static_cast<uint32_t>(tris.size());
 
-    auto ctx = std::make_shared<rtq::Context>(0);
+    // ---- the reference's call pattern through cr::BVH on the default context
+    {
+        ReferenceStyleScene scene;
+        scene._NbTriangles = n;
+        scene.bind(tris, models);
+        const std::vector<cr::BVH_NodeGPU> walked = scene.nodesForGpu(scene._BVH);
+        const std::vector<cr::BVH_NodeGPU> exported = scene._BVH->preorder();
+        CHECK(sizeof(cr::BVH_NodeGPU) == 48 && walked.size() == 2 * n - 1 && exported.size() == walked.size());
+        for (size_t i = 0; i < walked.size(); ++i) {
+            const cr::BVH_NodeGPU &a = walked[i], &b = exported[i];
+            CHECK(a._TriangleId == b._TriangleId && a._LeftChild == b._LeftChild && a._RightChild == b._RightChild);
+            for (int k = 0; k < 3; ++k)
+                CHECK(a._BoundingBox._Min[k] == b._BoundingBox._Min[k] && a._BoundingBox._Max[k] == b._BoundingBox._Max[k]);
+        }
+        CHECK(walked[0]._LeftChild == 1 && scene._BVH->_InternalStruct._TriangleIndices.size() == n);
+        CHECK(cr::AABB::getSurfaceArea(walked[0]._BoundingBox) == 2.0f);   // the unit square in y = 0
+        CHECK(walked[0]._BoundingBox._Min.x == 0.0f && walked[0]._BoundingBox._Max[2] == 1.0f);
+        for (const rtq_hit& h : scene._BVH->gpu().trace(std::vector<rtq_ray>{{{0.3f, 1.0f, 0.6f}, 0.0f,
+                                                                               {0.0f, -1.0f, 0.0f}, INFINITY}}))
+            CHECK(h.id >= 0 && h.t == 1.0f);
+    }
+
+    auto ctx = cr::defaultContext();
     rtq::BVH bvh(ctx, n, tris, models);
     const rtq::BVHParams& p = bvh._InternalStruct;
     CHECK(p._NbTriangles == n);

@@ -1,6 +1,7 @@
 """Host logic of bench.py, no GPU: the one-GPU guard, the reader of the
-committed HBM-traffic summary, and the comparison behind the `exact` flags
-(closest-hit records bit for bit, any-hit records by occlusion only)."""
+committed traffic summary and the roofline built from it, the rocprof kernel
+labels, and the comparison behind the `exact` flags (closest-hit records bit
+for bit, any-hit records by occlusion only)."""
 import json
 import subprocess
 import sys
@@ -22,22 +23,46 @@ def test_more_than_one_gpu_exits_with_a_message():
     assert "one GPU" in msg["error"]
 
 
-def test_read_traffic(tmp_path):
+def test_read_profile_and_roofline(tmp_path):
     f = tmp_path / "t.txt"
-    f.write_text("# comment\ntraffic_bytes_per_launch k_a: 123\ntraffic_bytes_per_launch k_b: 4.5e9\n")
-    assert bench.read_traffic(str(f), "k_a") == 123.0
-    assert bench.read_traffic(str(f), "k_b") == 4.5e9
-    assert bench.read_traffic(str(f), "k_c") is None
-    assert bench.read_traffic(str(tmp_path / "missing.txt"), "k_a") is None
+    f.write_text(f"# comment\nkernel_sources_sha256: {bench.kernel_digest()}\n"
+                 "hbm_bytes_per_launch k_a: 2000000000\nl2_bytes_per_launch k_a: 9.8304e9\n")
+    p = bench.read_profile(str(f))
+    assert p["fresh"] and p["hbm"]["k_a"] == 2e9 and p["l2"]["k_a"] == 9.8304e9
+    r = bench.roofline_of(p, "k_a", 1.0, 17_000_000_000, 96)
+    assert r["achieved"] == 2000.0 and r["frac"] == 0.25 and r["traffic"] == 2e9
+    assert r["algorithmic"]["GB_s"] == 17000.0 and r["l2"]["GB_s"] == 9830.4
+    assert 0 < r["l2"]["frac"] < 1
+    # a kernel the summary does not hold, and a summary of other kernel sources
+    assert bench.roofline_of(p, "k_b", 1.0, 1, 96)["frac"] is None
+    f.write_text("kernel_sources_sha256: 0000\nhbm_bytes_per_launch k_a: 2e9\n")
+    stale = bench.read_profile(str(f))
+    assert not stale["fresh"]
+    r = bench.roofline_of(stale, "k_a", 1.0, 1, 96)
+    assert r["frac"] is None and r["traffic_source"].startswith("stale")
+    assert not bench.read_profile(str(tmp_path / "missing.txt"))["fresh"]
+
+
+def test_kernel_labels():
+    assert bench.kernel_label("void rtq::(anonymous namespace)::k_trace<0, 1, false, 0, 0>(rtq::x)") == \
+        "k_trace<watertight,any,binary>"
+    assert bench.kernel_label("rtq::(anonymous namespace)::k_trace_occ<0, 2, false, 1, 2, 6>(x)") == \
+        "k_trace<watertight,per-item,wide,batch>"
+    assert bench.kernel_label("k_trace_occ<0, 0, false, 1, 0, 7>") == "k_trace<watertight,closest,wide>"
+    assert bench.kernel_label("k_trace<0, 0, true, 0, 0>") is None   # counter instantiation
+    assert bench.kernel_label("__amd_rocclr_copyBuffer") is None
 
 
 def test_committed_traffic_covers_every_reported_kernel():
-    path = str(ROOT / "profiles" / "traffic_terrain_trace.txt")
+    """The committed summary was measured on the current kernel sources and
+    holds HBM and L2 bytes for every kernel the bench reports; its HBM rate
+    for the headline launch stays below the HBM peak."""
+    prof = bench.read_profile(str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
+    assert prof["fresh"], f"re-profile: summary digest {prof['digest']} != sources {bench.kernel_digest()}"
     names = [f"k_trace<watertight,{m},{l}>" for m in ("any", "closest") for l in ("binary", "wide")]
-    names += [f"k_trace<watertight,per-item,{l},batch>" for l in ("binary", "wide", "learned")]
+    names += [f"k_trace<watertight,per-item,{l},batch>" for l in ("binary", "wide")]
     for n in names:
-        v = bench.read_traffic(path, n)
-        assert v is not None and v > 0, n
+        assert prof["hbm"].get(n, 0) > 0 and prof["l2"].get(n, 0) > 0, n
 
 
 def _hits(ids, ts):

@@ -29,6 +29,40 @@ def test_morton_codes(rtq_ctx, n):
         sc.release()
 
 
+def _flat_scene(n, seed, fix):
+    """Triangles whose vertices share the coordinates in `fix` (axis -> value):
+    the scene box is flat along those axes."""
+    rng = np.random.default_rng(seed)
+    v = [rng.uniform(0.0, 1.0, (n, 3)) for _ in range(3)]
+    for p in v:
+        for axis, value in fix.items():
+            p[:, axis] = value
+    return scenes.triangles_from_vertices(*v), scenes.identity_models()
+
+
+@pytest.mark.parametrize("case", ["y_branch", "z_branch", "single_point"])
+def test_cube_axis_branches_on_gpu(rtq_ctx, case):
+    """getCircumscribedCube takes X whenever distX > 0 (bvh.cpp:263-272); a
+    scene flat in x takes the Y branch, one flat in x and y the Z branch, and a
+    scene of one repeated point the Z branch with a zero-size cube.  Codes and
+    the built tree equal the oracle's in each case."""
+    fix = {"y_branch": {0: 0.3}, "z_branch": {0: 0.3, 1: 0.7}, "single_point": {0: 0.3, 1: 0.7, 2: 0.1}}[case]
+    tris, models = _flat_scene(700, seed=len(case), fix=fix)
+    sc = rtq_ctx.upload(tris, models)
+    try:
+        for flags in (0, RTQ_BUILD_REF_PADDED_BOUNDS):
+            got = sc.morton_codes(flags)
+            want = oracle.morton_codes(tris, models, pad_origin=bool(flags))
+            assert np.array_equal(got, want), (case, flags)
+        if case != "single_point":
+            assert len(np.unique(want)) > 100   # the branch spreads the codes over the cube
+    finally:
+        sc.release()
+    if case != "single_point":
+        sc, _, _ = _check_build(rtq_ctx, tris, models)
+        sc.release()
+
+
 def _check_build(rtq_ctx, tris, models, flags=0, radius=16, leaf=4):
     n = len(tris)
     sc = rtq_ctx.upload(tris, models)

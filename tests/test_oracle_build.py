@@ -76,6 +76,17 @@ def test_cube_axis_branches():
         assert codes[k] == (interleave_bits(q[0]) << 2) | (interleave_bits(q[1]) << 1) | interleave_bits(q[2])
 
 
+def test_cube_z_branch():
+    # distX == distY == 0: the Z branch (bvh.cpp:269-271), x and y grow to distZ
+    tris = np.concatenate([_tri((1, 2, 0), (1, 2, 4), (1, 2, 2)), _tri((1, 2, 1), (1, 2, 3), (1, 2, 3))])
+    codes = oracle.morton_codes(tris, scenes.identity_models())
+    # cube: x [-1, 3], y [0, 4], z [0, 4]
+    for k, t in enumerate(tris):
+        c = (t["p0"][:3].astype(np.float64) + t["p1"][:3] + t["p2"][:3]) / 3.0
+        q = np.clip(np.floor((c - np.array([-1.0, 0.0, 0.0])) / 4.0 * 1024.0), 0, 1023).astype(int)
+        assert codes[k] == (interleave_bits(q[0]) << 2) | (interleave_bits(q[1]) << 1) | interleave_bits(q[2])
+
+
 def test_model_matrix_centroid():
     # translation-only model: codes equal those of the translated triangles
     base = scenes.soup(50, seed=2)

@@ -31,7 +31,7 @@ extern "C" {
 #pragma GCC visibility push(default)
 #endif
 
-#define RTQ_ABI_VERSION 1
+#define RTQ_ABI_VERSION 2
 
 typedef struct rtq_ctx rtq_ctx;
 
@@ -128,14 +128,17 @@ void rtq_build_params_default(rtq_build_params* p);
  * Temporaries come from the context's scratch, which keeps its largest size. */
 rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params* params);
 
-/* Refit: the built tree's topology kept (PLOC merges, leaf order, node
- * indices), every box recomputed bottom-up for the scene's current vertices
- * (leaf = triangle box, bvh.cpp:383-399; parent = merge of its children,
- * bvh.cpp:401-413), then the reference node array, the traversal nodes and
- * triangles and the compressed wide nodes rewritten in their buffers, so batch
- * graphs captured before stay valid.  Learned order words are cleared.  Hit
- * records stay exact (they never depend on the tree); traversal cost grows as
- * the vertices move away from the ones the tree was built for. */
+/* Refit: the built tree's topology kept (PLOC merges, leaf order, binary
+ * traversal node indices), every box recomputed bottom-up for the scene's
+ * current vertices (leaf = triangle box, bvh.cpp:383-399; parent = merge of its
+ * children, bvh.cpp:401-413), then the reference node array, the traversal
+ * nodes and triangles and the compressed wide nodes rewritten in their
+ * buffers, so batch graphs captured before stay valid.  The 8-wide collapse is
+ * recomputed from the new boxes, so the 8-wide node count and depth may change
+ * (never beyond the binary traversal node count and depth, which bound its
+ * buffer and stacks).  Learned order words are cleared.  Hit records stay exact
+ * (they never depend on the tree); traversal cost grows as the vertices move
+ * away from the ones the tree was built for. */
 rtq_status rtq_bvh_refit(rtq_ctx* ctx, uint32_t scene_id);
 
 typedef struct rtq_bvh_info {
@@ -152,6 +155,9 @@ typedef struct rtq_bvh_info {
     uint32_t wide_depth;         /* levels of the 8-wide tree */
     uint32_t order_modes;        /* learned child order in the nodes (rtq_order_apply):
                                     bit 0 closest-hit decisions, bit 1 any-hit decisions */
+    uint64_t scratch_spill_bytes;/* temporaries of the last build that did not fit the
+                                    context scratch it reserved up front and were allocated
+                                    on their own (0 normally; a later build reserves them) */
 } rtq_bvh_info;
 
 rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out);
@@ -173,6 +179,15 @@ rtq_status rtq_bvh_export_ref(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_node* nod
  * buffer may be NULL; *n_nodes receives the node count. */
 rtq_status rtq_bvh_export_trav(rtq_ctx* ctx, uint32_t scene_id, void* nodes, uint32_t* ranges,
                                uint64_t capacity, uint64_t* n_nodes);
+
+/* The compressed 8-wide nodes (no counterpart in the reference), 128 B each
+ * in level order from the root: [ox oy oz meta] with meta = ex | ey << 8 |
+ * ez << 16 | valid << 24, then the 8-bit child planes [qlo_x[0..7] qhi_x[0..7]]
+ * for x, y and z, then the 8 child refs (leaf: bit 31 set, first leaf-order
+ * triangle in bits 30..3, count - 1 in bits 2..0; node: its index) and 32 B of
+ * zeros.  A child box is o + q * 2^(e - 127) per axis.  Host buffer of
+ * `capacity` nodes (NULL: only *n_nodes); RTQ_ENOTBUILT without 8-wide nodes. */
+rtq_status rtq_bvh_export_wide(rtq_ctx* ctx, uint32_t scene_id, void* nodes, uint64_t capacity, uint64_t* n_nodes);
 
 /* The build's cluster arrays in the form of BVH_Params (bvh.hpp:44-63):
  * cluster c < N is the leaf for sorted position c, c >= N are merges in the

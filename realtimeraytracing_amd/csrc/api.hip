@@ -512,6 +512,23 @@ rtq_status rtq_bvh_info_get(rtq_ctx* ctx, uint32_t scene_id, rtq_bvh_info* out) 
     out->n_wide_nodes = s->n_wide;
     out->order_modes = s->ordered ? ((s->order_closest ? 1u : 0u) | (s->order_any ? 2u : 0u)) : 0u;
     out->wide_depth = s->wide_depth;
+    out->scratch_spill_bytes = s->scratch_spill;
+    return RTQ_OK;
+}
+
+rtq_status rtq_bvh_export_wide(rtq_ctx* ctx, uint32_t scene_id, void* nodes, uint64_t capacity, uint64_t* n_nodes) {
+    if (!ctx) return RTQ_EINVAL;
+    Scene* s = find_scene(ctx, scene_id);
+    if (!s) return fail(ctx, RTQ_EINVAL, "bvh_export_wide: unknown scene %u", scene_id);
+    if (!s->built) return fail(ctx, RTQ_ENOTBUILT, "bvh_export_wide: scene %u has no BVH", scene_id);
+    if (!s->wide) return fail(ctx, RTQ_ENOTBUILT, "bvh_export_wide: scene %u has no 8-wide nodes", scene_id);
+    if (n_nodes) *n_nodes = s->n_wide;
+    if (!nodes) return RTQ_OK;
+    if (capacity < s->n_wide) return fail(ctx, RTQ_ECAPACITY, "bvh_export_wide: need %u nodes", s->n_wide);
+    RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    RTQ_HIP(ctx, hipMemcpyAsync(nodes, s->wide, sizeof(float4) * 8ull * s->n_wide, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    RTQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RTQ_OK;
 }
 

@@ -1250,10 +1250,14 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     reset_build(s);
     s->params = prm;
     s->n_clusters = nc;
-    // size the context's scratch for this build's temporaries up front (they
-    // come to about 120 B per triangle: sort and PLOC lists 64, placement 40,
-    // 8-wide collapse 16), so a first build does not allocate them one by one
-    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, (size_t)160 * n + ((size_t)1 << 20));
+    // size the context's scratch for this build's temporaries up front, so a
+    // first build does not allocate them one by one.  Alive together, per
+    // triangle: sort keys and values and the PLOC lists 64 B, placement 32 B;
+    // with 8-wide nodes also the collapse cost table 36 B and the collapse
+    // buffers 52 B per binary traversal node (one per triangle with leaves of
+    // one triangle); 1 MiB for scans and 256-B rounding
+    const size_t per_tri = prm.node_width == 8 ? 64 + 32 + 36 + 52 : 64 + 32;
+    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + ((size_t)1 << 20));
     rtq_status rs = arena_reserve(ctx);
     if (rs != RTQ_OK) return rs;
 
@@ -1336,6 +1340,8 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     s->build_ms = ms;
     s->built = true;
     s->generation = ++ctx->builds;
+    // temporaries that did not fit the scratch reserved above raised its peak
+    s->scratch_spill = ctx->arena.peak > ctx->arena.cap ? ctx->arena.peak - ctx->arena.cap : 0;
     return RTQ_OK;
 }
 

@@ -92,6 +92,7 @@ struct Scene {
     bool stale = false;
     float box_pad = 0.0f;
     float build_ms = 0.0f;
+    uint64_t scratch_spill = 0;         // bytes of the last build's temporaries beyond the reserved scratch
     // set by each build from the context's counter, 0 while unbuilt: a batch
     // graph records it and refuses to replay over buffers of another build
     uint64_t generation = 0;

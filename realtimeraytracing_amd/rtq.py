@@ -57,7 +57,7 @@ EXPORTED = [
     "rtq_order_model_create", "rtq_order_model_destroy", "rtq_order_model_log", "rtq_order_model_clear_samples",
     "rtq_order_model_train", "rtq_order_model_apply", "rtq_order_model_save", "rtq_order_model_load",
     "rtq_order_model_info", "rtq_shade", "rtq_bvh_overlay", "rtq_obj_load", "rtq_obj_parse",
-    "rtq_scene_update", "rtq_bvh_refit", "rtq_trace_noalloc",
+    "rtq_scene_update", "rtq_bvh_refit", "rtq_trace_noalloc", "rtq_bvh_export_wide",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -88,7 +88,8 @@ class BvhInfo(ctypes.Structure):
                 ("ploc_iterations", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
                 ("box_pad", ctypes.c_float), ("build_ms", ctypes.c_float),
                 ("node_width", ctypes.c_uint32), ("n_wide_nodes", ctypes.c_uint32),
-                ("wide_depth", ctypes.c_uint32), ("order_modes", ctypes.c_uint32)]
+                ("wide_depth", ctypes.c_uint32), ("order_modes", ctypes.c_uint32),
+                ("scratch_spill_bytes", ctypes.c_uint64)]
 
 
 class TrainParams(ctypes.Structure):
@@ -137,6 +138,7 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_bvh_build": (st, [vp, u32, ctypes.POINTER(BuildParams)]),
         "rtq_bvh_info_get": (st, [vp, u32, ctypes.POINTER(BvhInfo)]),
         "rtq_bvh_export_ref": (st, [vp, u32, vp, u64, ctypes.POINTER(u64)]),
+        "rtq_bvh_export_wide": (st, [vp, u32, vp, u64, ctypes.POINTER(u64)]),
         "rtq_bvh_export_clusters": (st, [vp, u32, u64, vp, vp, vp, vp, vp, vp, vp]),
         "rtq_morton_codes": (st, [vp, u32, u32, vp]),
         "rtq_sort_pairs": (st, [vp, vp, vp, u64, u32, vp]),
@@ -442,6 +444,15 @@ class Scene:
                                        ("boxes", "triangle_id", "left", "right", "parent",
                                         "sorted_indices", "sorted_codes"))))
         return out
+
+    def export_wide(self) -> np.ndarray:
+        """The compressed 8-wide nodes, n x 32 uint32 (128 B each, level order)."""
+        n = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_wide(self.ctx.h, self.id, None, 0, ctypes.byref(n)))
+        nodes = np.zeros((n.value, 32), np.uint32)
+        self.ctx.check(self.ctx.lib.rtq_bvh_export_wide(self.ctx.h, self.id, _ptr(nodes), n.value,
+                                                        ctypes.byref(n)))
+        return nodes
 
     def export_trav(self) -> tuple[np.ndarray, np.ndarray]:
         """Binary traversal nodes (n x 16 float32; words 12-14 hold u32 refs and the

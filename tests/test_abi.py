@@ -79,7 +79,7 @@ def test_headers_compile_and_sizes(tmp_path, compiler, std):
 
 def test_argument_errors_without_device():
     lib = load_library()
-    assert lib.rtq_abi_version() == 1
+    assert lib.rtq_abi_version() == 2
     assert lib.rtq_status_string(RTQ_EINVAL) == b"invalid argument"
     assert lib.rtq_last_error(None) == b"null context"
     p = BuildParams()

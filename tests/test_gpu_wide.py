@@ -151,24 +151,87 @@ def _collapse_restated(cl, n_tris, max_leaf, tri_cost=np.float32(0.6)):
         assert len(out) <= 8
         return out
 
-    items, n_nodes, levels = [nc - 1], 0, 0
+    # first leaf-order triangle of every cluster (leaves in left-to-right order)
+    first = np.zeros(nc, np.int64)
+    for c in range(nc - 1, n_tris - 1, -1):
+        first[left[c]] = first[c]
+        first[right[c]] = first[c] + count[left[c]]
+
+    # per node in level order: its children left to right, ("leaf", first
+    # triangle, count) or ("node", -, triangles below)
+    items, levels, nodes = [nc - 1], 0, []
     while items:
-        n_nodes += len(items)
-        items = [x for c in items for x in children(c) if left[x] != none and count[x] > max_leaf]
+        nxt = []
+        for c in items:
+            kids = []
+            for x in children(c):
+                if left[x] != none and count[x] > max_leaf:
+                    kids.append(("node", -1, int(count[x])))
+                    nxt.append(x)
+                else:
+                    kids.append(("leaf", int(first[x]), int(count[x])))
+            nodes.append(kids)
+        items = nxt
         levels += 1
-    return n_nodes, levels
+    return nodes, levels
 
 
-@pytest.mark.parametrize("n,leaf", [(1500, 2), (3000, 4)])
+def _wide_children(raw, ccount_of_node):
+    """The GPU's 8-wide nodes (n x 32 u32) as the restatement lists them."""
+    out = []
+    for k in range(len(raw)):
+        valid = int(raw[k, 3]) >> 24
+        kids = []
+        for j in range(8):
+            if not (valid >> j) & 1:
+                continue
+            ref = int(raw[k, 16 + j])
+            if ref & 0x80000000:
+                kids.append(("leaf", (ref >> 3) & 0x0FFFFFFF, (ref & 7) + 1))
+            else:
+                kids.append(("node", -1, ccount_of_node[ref]))
+        out.append(kids)
+    return out
+
+
+@pytest.mark.parametrize("n,leaf", [(1500, 2), (3000, 4), (4000, 1)])
 def test_wide_collapse_follows_cost_table(rtq_ctx, n, leaf):
-    """Node count and depth of the 8-wide tree equal those of the collapse
-    restated on the CPU from the exported clusters (hits cannot show a wrong
+    """The 8-wide tree equals the collapse restated on the CPU from the
+    exported clusters, node by node in level order: each child's kind, leaf
+    triangle range and subtree size, in slot order (hits cannot show a wrong
     cost table: any tree shape gives the same hits)."""
     tris, models = scenes.soup(n, seed=40 + n), scenes.identity_models()
     sc = rtq_ctx.upload(tris, models)
     try:
         info = sc.build(16, leaf, 0, node_width=8)
-        want = _collapse_restated(sc.export_clusters(), n, leaf)
-        assert (info.n_wide_nodes, info.wide_depth) == want
+        want, levels = _collapse_restated(sc.export_clusters(), n, leaf)
+        assert (info.n_wide_nodes, info.wide_depth) == (len(want), levels)
+        # subtree sizes of the GPU's nodes, from their own children (bottom-up)
+        raw = sc.export_wide()
+        size = [0] * len(raw)
+        for k in range(len(raw) - 1, -1, -1):
+            valid = int(raw[k, 3]) >> 24
+            for j in range(8):
+                ref = int(raw[k, 16 + j])
+                if (valid >> j) & 1:
+                    size[k] += (ref & 7) + 1 if ref & 0x80000000 else size[ref]
+        got = _wide_children(raw, size)
+        assert got == want
+        # the next level's nodes are numbered consecutively, in slot order
+        refs = [int(raw[k, 16 + j]) for k in range(len(raw)) for j in range(8)
+                if (int(raw[k, 3]) >> 24 >> j) & 1 and not int(raw[k, 16 + j]) & 0x80000000]
+        assert refs == list(range(1, len(raw)))
     finally:
+        sc.release()
+
+
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_first_build_fits_reserved_scratch(leaf):
+    """A context's first 8-wide build takes its temporaries from the scratch it
+    reserves up front (none allocated on their own)."""
+    from realtimeraytracing_amd import Context
+    with Context(0) as ctx:
+        sc = ctx.upload(*scenes.soup_scene(200_000, seed=leaf))
+        info = sc.build(16, leaf, 0, node_width=8)
+        assert info.scratch_spill_bytes == 0
         sc.release()

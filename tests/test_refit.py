@@ -97,6 +97,11 @@ def test_refit_moved_vertices(rtq_ctx, width):
         assert all(np.array_equal(got[k], cl[k]) for k in ("triangle_id", "left", "right", "parent"))
         ref = oracle.flatten_preorder(n, dict(cl, boxes=boxes))
         assert hits_equal(sc.export_ref(), ref), first_diff(sc.export_ref(), ref)
+        # the 8-wide collapse is recomputed from the new boxes: its node count
+        # and depth may change, within the binary traversal tree's
+        info = sc.info()
+        if width == 8:
+            assert 0 < info.n_wide_nodes <= info.n_trav_nodes and 0 < info.wide_depth <= info.depth + 1
         # hits equal brute force over the new vertices, closest and any hit
         rays = rays_for()
         v9 = oracle.world_vertices(new, models)

@@ -19,8 +19,9 @@ here too, `exact`):
               hipGraph, settings (refill, chunk, node layout) from the
               autotuner
   learned     the batched form with the learned child order (model trained
-              on this scene's own traversal logs before timing; any-hit
-              decisions only, closest-hit rays keep the nearer child)
+              on this scene's own traversal logs before timing): on 8-wide
+              nodes a learned first child per node, ray octant and mode, on
+              binary nodes a learned child order
 `value` is the fastest form; `detail.forms` has every form.
 
 Further workloads in `detail` (not the headline):
@@ -32,8 +33,9 @@ one graph-replayed launch against 64 launches, results compared.
 `detail.occluded` measures the learned-ordering workload: the same terrain
 plus 2,097,152 random-soup triangles over the unit cube, 2048 x 2048 primary
 hits -> 4M diffuse + 16M shadow rays (nearly all occluded): plain, 8-wide
-compressed nodes, learned order (two launches) and learned order batched in
-one graph-replayed launch, with the same exactness check.
+compressed nodes, learned order on binary nodes and on 8-wide nodes (two
+launches each) and learned order batched in one graph-replayed launch, with
+the same exactness check.
 
 Prints one JSON line with the metric, a roofline object for the dominant
 trace kernel of the best form (HBM bytes per launch from the committed,
@@ -64,6 +66,9 @@ RAY_BYTES = 32
 HIT_BYTES = 16
 BINARY_NODE_BYTES = 64   # one binary node per visit
 WIDE_NODE_BYTES = 96     # an 8-wide visit reads 96 B of its 128-B slot
+LEARNED_WIDE_NODE_BYTES = 104   # with the learned first-child table (bytes 96-103)
+NODE_BYTES = {"binary": BINARY_NODE_BYTES, "wide": WIDE_NODE_BYTES, "learned": BINARY_NODE_BYTES,
+              "learned_wide": LEARNED_WIDE_NODE_BYTES}
 # sources the traversal kernels' memory traffic depends on; the committed
 # traffic summary records their digest and is ignored when it differs
 KERNEL_SOURCES = ("trace.hip", "wide.hip", "build.hip", "batch.hip", "rtq_internal.hpp")
@@ -78,7 +83,7 @@ def kernel_label(rocprof_name: str):
         return None
     leaf = "watertight" if m.group(1) == "0" else "refmt"
     mode = {"0": "closest", "1": "any", "2": "per-item"}[m.group(2)]
-    layout = {"0": "binary", "1": "wide", "2": "learned"}[m.group(4)]
+    layout = {"0": "binary", "1": "wide", "2": "learned", "3": "learned_wide"}[m.group(4)]
     return f"k_trace<{leaf},{mode},{layout}" + (",batch>" if m.group(5) != "0" else ">")
 
 
@@ -447,9 +452,11 @@ def main() -> int:
                      "spec_slack": tr.best.spec_slack, "leaf_slack": tr.best.leaf_slack, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
                      "default_ms": round(tr.default_ms, 4)}
             if form == "learned" and wl.sc.info().order_modes:
-                # learned decisions: the batch traces binary nodes with the learned order
-                lname = "learned"
-                counters_by_layout["learned"] = wl.counters(RTQ_TRACE_LEARNED | RTQ_TRACE_BINARY, BINARY_NODE_BYTES)
+                # learned decisions: the batch traces 8-wide nodes with their
+                # learned first child, or binary nodes with the learned order
+                lname = "learned_wide" if tr.best.prefer_wide else "learned"
+                lflags = RTQ_TRACE_LEARNED | (0 if tr.best.prefer_wide else RTQ_TRACE_BINARY)
+                counters_by_layout[lname] = wl.counters(lflags, NODE_BYTES[lname])
             else:
                 # the batched form, or learned order without decisions (the plain
                 # order), on the layout the autotuner chose
@@ -468,7 +475,7 @@ def main() -> int:
         got = wl.hits()
         if reference is None:
             reference = got
-        node_bytes = WIDE_NODE_BYTES if lname == "wide" else BINARY_NODE_BYTES
+        node_bytes = NODE_BYTES[lname]
         results[form] = {
             "Mrays_s": wl.live_rays / (ms_per_step * 1e-3) / 1e6,
             "ms_per_step": ms_per_step, "diffuse_ms": diff_ms, "shadow_ms": shad_ms,
@@ -672,13 +679,13 @@ def main() -> int:
         del otris
         oforms = {}
         oref = None
-        for form in ("plain", "compressed", "learned", "learned_batched"):
+        for form in ("plain", "compressed", "learned_binary", "learned", "learned_batched"):
             flags = 0 if form == "compressed" else RTQ_TRACE_BINARY
             setup = None
-            if form == "learned":
+            if form == "learned_binary":
                 setup = train_order(ctx, owl, sp)
             if form.startswith("learned"):
-                flags = RTQ_TRACE_LEARNED
+                flags = RTQ_TRACE_LEARNED | (RTQ_TRACE_BINARY if form == "learned_binary" else 0)
             if form == "learned_batched":
                 ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
                               default_params.spec_slack, default_params.leaf_slack)
@@ -694,7 +701,8 @@ def main() -> int:
             got = owl.hits()
             if oref is None:
                 oref = got
-            c = owl.counters(flags, WIDE_NODE_BYTES if form == "compressed" else BINARY_NODE_BYTES)
+            c = owl.counters(flags, {"compressed": WIDE_NODE_BYTES, "learned": LEARNED_WIDE_NODE_BYTES}.get(
+                form, BINARY_NODE_BYTES))
             split = form != "learned_batched"
             oforms[form] = {
                 "Mrays_s": round(owl.live_rays / (ms_per_step * 1e-3) / 1e6, 2),
@@ -709,6 +717,8 @@ def main() -> int:
                         f"{args.occluded_res}x{args.occluded_res} primary hits -> 1 diffuse + 4 shadow rays each",
             "rays_per_step": owl.live_rays, "forms": oforms,
             "learned_vs_plain": round(oforms["learned"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
+            "learned_vs_compressed": round(oforms["learned"]["Mrays_s"] / oforms["compressed"]["Mrays_s"], 3),
+            "learned_binary_vs_plain": round(oforms["learned_binary"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
             "compressed_vs_plain": round(oforms["compressed"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
             "learned_batched_vs_plain": round(oforms["learned_batched"]["Mrays_s"] / oforms["plain"]["Mrays_s"], 3),
             "shadow_speedup": round(oforms["plain"]["shadow_ms"] / oforms["learned"]["shadow_ms"], 3),

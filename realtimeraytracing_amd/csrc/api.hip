@@ -54,6 +54,8 @@ void reset_build(Scene* s) {
     release(s->nodes);
     release(s->node_aux);
     release(s->wide);
+    release(s->wide_trav);
+    s->wide_ordered = false;
     release(s->d_passes);
     s->small_from = 0;
     s->n_top = 0;

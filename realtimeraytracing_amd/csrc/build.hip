@@ -1216,7 +1216,7 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
         RTQ_HIP(ctx, hipGetLastError());
     }
     if (s->params.node_width == 8) {
-        rtq_status rs = build_wide(ctx, s, fl, st);
+        rtq_status rs = build_wide(ctx, s, fl, pos, n == 1 ? nullptr : bidx, st);
         if (rs != RTQ_OK) return rs;
     }
     rtq_status rs = write_top(ctx, s, st);

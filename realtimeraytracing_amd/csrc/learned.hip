@@ -592,7 +592,76 @@ __global__ void k_order_clear(float4* __restrict__ nodes, uint32_t n_nodes) {
     if (i < n_nodes) nodes[4ull * i + 3].z = 0.0f;
 }
 
+// Learned first child of every 8-wide node, per (octant, mode): from the
+// binary node of the wide node's root cluster, follow the binary decisions
+// (the child order word's decided child) while they are confident, until the
+// path reaches one of the wide node's children; that slot is entry
+// octant + 8 * any_hit of the table (8 | slot; 0 where a decision is missing).
+// The children of a wide node are the frontier of the binary subtree below its
+// root, so every decided path reaches one within 7 steps.
+__global__ __launch_bounds__(256) void k_wide_order(const float4* __restrict__ nodes, float4* __restrict__ wide,
+                                                    const uint32_t* __restrict__ wide_trav, uint32_t n_wide,
+                                                    uint32_t n_trav) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_wide) return;
+    const float4* wn = wide + 8ull * k;
+    const uint32_t valid = __float_as_uint(wn[0].w) >> 24;
+    const float4 a = wn[4], b = wn[5];
+    const uint32_t refs[8] = {__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(a.z), __float_as_uint(a.w),
+                              __float_as_uint(b.x), __float_as_uint(b.y), __float_as_uint(b.z), __float_as_uint(b.w)};
+    // each child as the binary layout refers to it: a leaf ref is the same in
+    // both layouts, an internal child is the binary node of its root cluster
+    uint32_t as_binary[8];
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t r = refs[j];
+        as_binary[j] = !((valid >> j) & 1u) ? kNone : ((r & kLeafBit) ? r : wide_trav[r]);
+    }
+    uint32_t table[2] = {0u, 0u};
+    const uint32_t root = wide_trav[k];
+    for (uint32_t e = 0; e < 16 && root < n_trav; ++e) {
+        uint32_t x = root, entry = 0;
+        for (int step = 0; step < 8; ++step) {
+            const float4 n3 = nodes[4ull * x + 3];
+            const uint32_t ow = __float_as_uint(n3.z);
+            if (!((ow >> (16 + e)) & 1u)) break;   // no confident decision
+            const uint32_t child = ((ow >> e) & 1u) ? __float_as_uint(n3.x) : __float_as_uint(n3.y);
+            int slot = -1;
+            for (int j = 0; j < 8; ++j)
+                if (as_binary[j] == child) slot = j;
+            if (slot >= 0) {
+                entry = 8u | (uint32_t)slot;
+                break;
+            }
+            if ((child & kLeafBit) || child >= n_trav) break;   // not inside this node's subtree
+            x = child;
+        }
+        table[e >> 3] |= entry << (4 * (e & 7));
+    }
+    wide[8ull * k + 6] = make_float4(__uint_as_float(table[0]), __uint_as_float(table[1]), 0.0f, 0.0f);
+}
+
+__global__ void k_wide_order_clear(float4* __restrict__ wide, uint32_t n_wide) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n_wide) wide[8ull * k + 6] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace
+
+rtq_status write_wide_order(rtq_ctx* ctx, Scene* s, bool clear, hipStream_t st) {
+    s->wide_ordered = false;
+    if (!s->wide || s->n_wide == 0) return RTQ_OK;
+    const uint32_t blocks = (s->n_wide + 255) / 256;
+    if (clear) {
+        hipLaunchKernelGGL(k_wide_order_clear, dim3(blocks), dim3(256), 0, st, s->wide, s->n_wide);
+    } else {
+        hipLaunchKernelGGL(k_wide_order, dim3(blocks), dim3(256), 0, st, s->nodes, s->wide, s->wide_trav, s->n_wide,
+                           s->n_trav);
+    }
+    RTQ_HIP(ctx, hipGetLastError());
+    s->wide_ordered = !clear;
+    return RTQ_OK;
+}
+
 }  // namespace rtq
 
 using namespace rtq;
@@ -773,6 +842,9 @@ rtq_status rtq_order_apply(rtq_ctx* ctx, uint32_t scene_id, const float* params_
     RTQ_HIP(ctx, hipGetLastError());
     // the LDS copy of the top of the tree carries the order words too
     rtq_status rs = write_top(ctx, s, st);
+    if (rs != RTQ_OK) return rs;
+    // the 8-wide nodes' first-child tables follow the binary decisions
+    rs = write_wide_order(ctx, s, params_dev == nullptr, st);
     if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     s->ordered = params_dev != nullptr;

@@ -77,6 +77,8 @@ struct Scene {
     float4* tris = nullptr;             // traversal triangles, leaf order, 3 float4 (48 B) each
     uint32_t n_trav = 0;
     float4* wide = nullptr;             // compressed 8-wide nodes, 8 float4 (128 B) each
+    uint32_t* wide_trav = nullptr;      // binary traversal node of every 8-wide node's root cluster
+    bool wide_ordered = false;          // learned first-child tables written in the 8-wide nodes
     uint32_t n_wide = 0;
     uint32_t wide_depth = 0;            // levels of the wide tree
     uint32_t depth = 0;
@@ -251,7 +253,13 @@ rtq_status trace_noalloc(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint
                          rtq_hit* hits, hipStream_t st);
 void scratch_free(TraceScratch* sc);
 // Compressed 8-wide nodes from the cluster arrays (fl: first leaf rank per cluster).
-rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st);
+// pos, bidx: pre-order position of every cluster and binary traversal index
+// of every position (null for a one-triangle scene, which has no binary node).
+rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, const uint32_t* pos, const uint32_t* bidx,
+                      hipStream_t st);
+// Learned first-child tables of the 8-wide nodes from the binary nodes' order
+// words (learned.hip); clear = zero them.
+rtq_status write_wide_order(rtq_ctx* ctx, Scene* s, bool clear, hipStream_t st);
 // New boxes for the kept topology after rtq_scene_update (build.hip).
 rtq_status refit_scene(rtq_ctx* ctx, Scene* s);
 // (Re)writes s->top from s->nodes: after the traversal nodes or their order

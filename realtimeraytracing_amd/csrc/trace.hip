@@ -243,9 +243,13 @@ __device__ __forceinline__ bool is_node_ref(uint32_t r) { return !(r & kLeafBit)
 // Node layouts a launch traces.
 enum Layout : int {
     kBinary = 0,   // 64-B two-child nodes, nearer child first
-    kWide = 1,     // 128-B compressed 8-wide nodes
-    kOrdered = 2,  // 64-B two-child nodes, learned child order where both are hit
+    kWide = 1,         // 128-B compressed 8-wide nodes
+    kOrdered = 2,      // 64-B two-child nodes, learned child order where both are hit
+    kWideOrdered = 3,  // 8-wide nodes, the learned first child where it is hit
 };
+
+__host__ __device__ constexpr bool wide_layout(int layout) { return layout == kWide || layout == kWideOrdered; }
+__host__ __device__ constexpr bool learned_layout(int layout) { return layout == kOrdered || layout == kWideOrdered; }
 
 // MODE: 0 closest hit, 1 any hit, 2 per batch item.  MULTI: 0 one ray set;
 // 1 several items (ray sets over any scenes), each lane takes its scene from
@@ -257,18 +261,18 @@ enum Layout : int {
 // occupancy hint).
 template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
 __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<(MULTI != 0)>& table) {
-    constexpr uint32_t kStackLds = LAYOUT == kWide ? kStackLdsWide : kStackLdsBinary;
+    constexpr uint32_t kStackLds = wide_layout(LAYOUT) ? kStackLdsWide : kStackLdsBinary;
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
     // the top of the binary tree, read from LDS instead of L2 by every ray
-    constexpr uint32_t kTopSlots = (LAYOUT == kWide || kTopNodes == 0) ? 1u : kTopNodes;
+    constexpr uint32_t kTopSlots = (wide_layout(LAYOUT) || kTopNodes == 0) ? 1u : kTopNodes;
     __shared__ float4 s_top[4 * kTopSlots];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     uint32_t* const lds = s_stack + tid;  // entry k at lds[k * kTraceThreads]: lanes on distinct banks
     uint32_t* const spill = a.spill + (size_t)blockIdx.x * kTraceThreads + tid;
     const size_t stride = a.spill_stride;
-    const uint32_t n_top = (LAYOUT == kWide || kTopNodes == 0) ? 0u : min(a.n_top, kTopSlots);
+    const uint32_t n_top = (wide_layout(LAYOUT) || kTopNodes == 0) ? 0u : min(a.n_top, kTopSlots);
     const uint32_t root = n_top ? kTopBit : 0u;
     if constexpr (MULTI)
         for (uint32_t i = tid; i < a.n_items; i += kTraceThreads) s_items[i] = table.item[i];
@@ -357,7 +361,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
     };
     // One internal-node visit: cur <- nearest hit child (farther ones pushed) or a pop.
     auto visit = [&]() {
-        if constexpr (LAYOUT == kWide) {
+        if constexpr (wide_layout(LAYOUT)) {
             const float4* nd = nodes_p + 8ull * cur;
             const float4 h = nd[0];
             const uint4 qx = bits4(nd[1]), qy = bits4(nd[2]), qz = bits4(nd[3]);
@@ -372,6 +376,15 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             const float by = __builtin_fmaf(h.y, iy, -oiy);
             const float bz = __builtin_fmaf(h.z, iz, -oiz);
             const uint32_t valid = meta >> 24;
+            // learned first child for the ray's octant and mode (table entry
+            // 8 | slot; lf = 8: none); dropped from the key list when hit
+            uint32_t lf = 8u;
+            if constexpr (LAYOUT == kWideOrdered) {
+                const uint2 tab = *reinterpret_cast<const uint2*>(nd + 6);
+                const uint32_t e = (((order_bit & 8u) ? tab.y : tab.x) & order_mask) >> (4u * (order_bit & 7u));
+                lf = (e & 8u) ? (e & 7u) : 8u;
+            }
+            bool lf_hit = false;
             // near / far planes by the direction's sign: with ix >= 0 the lower
             // plane is the nearer (q_lo <= q_hi and the FMA is monotonic, so
             // this equals min / max of the two distances exactly)
@@ -392,25 +405,29 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
                 const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest));
                 const bool hit = (tn <= __builtin_fmaf(fabsf(tf), kSlabRel, tf)) && ((valid >> j) & 1u);
+                const bool learned_first = LAYOUT == kWideOrdered && hit && lf == (uint32_t)j;
+                if (LAYOUT == kWideOrdered) lf_hit = lf_hit || learned_first;
                 // order key: entry distance (non-negative floats order as uints), slot in the low bits
-                key[j] = hit ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j) : 0xFFFFFFFFu;
+                key[j] = (hit && !learned_first) ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j)
+                                                 : 0xFFFFFFFFu;
             }
             if (MODE == 1 || (MODE == 2 && lane_any)) {
-                // any hit: nearest child first, the others in slot order
+                // any hit: nearest (or the learned) child first, the others in slot order
                 uint32_t m = min(min(min(key[0], key[1]), min(key[2], key[3])),
                                  min(min(key[4], key[5]), min(key[6], key[7])));
-                if (m == 0xFFFFFFFFu) {
+                if (!lf_hit && m == 0xFFFFFFFFu) {
                     cur = pop();
                     return;
                 }
-                const uint32_t first = m & 7u;
+                const uint32_t first = lf_hit ? lf : (m & 7u);
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
                     if (key[j] != 0xFFFFFFFFu && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
                 cur = slot_ref(c03, c47, first);
             } else {
+                // the learned child (not in the keys) goes first, the others nearest first
                 sort8(key);
-                if (key[0] == 0xFFFFFFFFu) {
+                if (!lf_hit && key[0] == 0xFFFFFFFFu) {
                     cur = pop();
                     return;
                 }
@@ -418,7 +435,12 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
 #pragma unroll
                 for (int j = 7; j >= 1; --j)
                     if (key[j] != 0xFFFFFFFFu) push(slot_ref(c03, c47, key[j] & 7u));
-                cur = slot_ref(c03, c47, key[0] & 7u);
+                if (lf_hit) {
+                    if (key[0] != 0xFFFFFFFFu) push(slot_ref(c03, c47, key[0] & 7u));
+                    cur = slot_ref(c03, c47, lf);
+                } else {
+                    cur = slot_ref(c03, c47, key[0] & 7u);
+                }
             }
         } else {
             float4 n0, n1, n2, n3;
@@ -549,7 +571,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                     const float4 r0 = rp[0], r1 = rp[1];
                     ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
                     dx = r1.x; dy = r1.y; dz = r1.z; tbest = r1.w;
-                    if constexpr (LAYOUT == kOrdered) {
+                    if constexpr (learned_layout(LAYOUT)) {
                         const bool any_ray = MODE == 1 || (MODE == 2 && lane_any);
                         order_bit = (dx < 0.0f ? 1u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 4u : 0u) |
                                     (any_ray ? 8u : 0u);
@@ -715,10 +737,20 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(W
 #ifndef RTQ_OCC_BATCH_WIDE_ONE
 #define RTQ_OCC_BATCH_WIDE_ONE 6
 #endif
+// The learned 8-wide kernels hold the first-child decode as well: 6 waves per
+// SIMD keep them free of spills.
+#ifndef RTQ_OCC_WIDE_ORDERED
+#define RTQ_OCC_WIDE_ORDERED 6
+#endif
+#ifndef RTQ_OCC_BATCH_WIDE_ORDERED_ONE
+#define RTQ_OCC_BATCH_WIDE_ORDERED_ONE 6
+#endif
 template <int LAYOUT, int MULTI>
 constexpr int occupancy_hint() {
-    if constexpr (MULTI == 0) return LAYOUT == kWide ? RTQ_OCC_WIDE : 0;
-    if constexpr (LAYOUT == kWide) return MULTI == 2 ? RTQ_OCC_BATCH_WIDE_ONE : RTQ_OCC_BATCH_WIDE;
+    if constexpr (LAYOUT == kWideOrdered && MULTI == 0) return RTQ_OCC_WIDE_ORDERED;
+    if constexpr (LAYOUT == kWideOrdered && MULTI == 2) return RTQ_OCC_BATCH_WIDE_ORDERED_ONE;
+    if constexpr (MULTI == 0) return wide_layout(LAYOUT) ? RTQ_OCC_WIDE : 0;
+    if constexpr (wide_layout(LAYOUT)) return MULTI == 2 ? RTQ_OCC_BATCH_WIDE_ONE : RTQ_OCC_BATCH_WIDE;
     return RTQ_OCC_BATCH;  // binary, with or without the learned order
 }
 
@@ -743,7 +775,7 @@ rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const 
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
     if (blocks == 0) blocks = 1;
-    constexpr uint32_t kStackLds = LAYOUT == kWide ? kStackLdsWide : kStackLdsBinary;
+    constexpr uint32_t kStackLds = wide_layout(LAYOUT) ? kStackLdsWide : kStackLdsBinary;
     const uint64_t column = depth_entries > kStackLds ? depth_entries - kStackLds : 0;  // spill words per thread
     if (sc.fixed && column > 0) {
         // a spill buffer that cannot grow: as many workgroups as it holds
@@ -800,7 +832,7 @@ template <int LEAF, int MODE, bool STATS>
 rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, int layout, hipStream_t st,
                          TraceScratch* fixed = nullptr) {
     const ItemTable<false> none{};
-    const bool wide = layout == kWide;
+    const bool wide = wide_layout(layout);
     a.nodes = wide ? s->wide : s->nodes;
     a.top = wide ? nullptr : s->top;
     a.n_top = wide ? 0u : s->n_top;
@@ -809,6 +841,8 @@ rtq_status launch_single(rtq_ctx* ctx, const Scene* s, TraceArgs a, uint32_t n, 
     // tree's own bound (at most 7 pushes per level) instead of the binary one
     const uint64_t depth = (fixed && wide) ? 7ull * s->wide_depth + 2 : stack_bound(s, wide);
     if (layout == kWide) return launch<LEAF, MODE, STATS, kWide, 0>(ctx, sc, depth, a, none, n, false, st);
+    if (layout == kWideOrdered)
+        return launch<LEAF, MODE, STATS, kWideOrdered, 0>(ctx, sc, depth, a, none, n, false, st);
     if (layout == kOrdered) return launch<LEAF, MODE, STATS, kOrdered, 0>(ctx, sc, depth, a, none, n, false, st);
     return launch<LEAF, MODE, STATS, kBinary, 0>(ctx, sc, depth, a, none, n, false, st);
 }
@@ -830,7 +864,11 @@ int pick_layout(rtq_ctx* ctx, const Scene* s, uint32_t flags, bool allow_wide = 
             fail(ctx, RTQ_ENOTBUILT, "trace: RTQ_TRACE_LEARNED needs order bits (rtq_order_apply)");
             return -1;
         }
-        if (learned_decisions(s, flags)) return kOrdered;
+        if (learned_decisions(s, flags))
+            return (allow_wide && s->wide != nullptr && s->wide_ordered && !(flags & RTQ_TRACE_BINARY) &&
+                    ctx->prefer_wide)
+                       ? kWideOrdered
+                       : kOrdered;
     }
     return (allow_wide && s->wide != nullptr && !(flags & RTQ_TRACE_BINARY) && ctx->prefer_wide) ? kWide : kBinary;
 }
@@ -955,11 +993,15 @@ rtq_status launch_batch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth, const Tr
                         int layout) {
     if (refmt) {
         if (layout == kWide) return launch<1, 2, false, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+        if (layout == kWideOrdered)
+            return launch<1, 2, false, kWideOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
         if (layout == kOrdered)
             return launch<1, 2, false, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
         return launch<1, 2, false, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
     }
     if (layout == kWide) return launch<0, 2, false, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (layout == kWideOrdered)
+        return launch<0, 2, false, kWideOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
     if (layout == kOrdered) return launch<0, 2, false, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
     return launch<0, 2, false, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
 }
@@ -977,12 +1019,19 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
     uint64_t total = 0, depth = 0;
     const uint32_t leaf_flags = items[0].flags & RTQ_TRACE_LEAF_REFMT;
     int layout = -1;
-    // items with learned decisions trace binary nodes, and so does the rest of
-    // the batch (plain and learned binary items share one kernel)
-    bool any_ordered = false;
+    // Items with learned decisions trace a learned layout, and so does the
+    // rest of the batch (a per-item order mask turns the decisions off for
+    // the others): 8-wide nodes when every item can trace them (8-wide nodes
+    // in its scene, their first-child tables for learned items, no
+    // RTQ_TRACE_BINARY, 8-wide nodes preferred), binary nodes otherwise.
+    bool any_learned = false, all_wide = true;
     for (uint32_t i = 0; i < n_items; ++i) {
         const Scene* s = find_scene(ctx, items[i].scene_id);
-        if (s && s->built && learned_decisions(s, items[i].flags)) any_ordered = true;
+        if (!s || !s->built) continue;   // reported below
+        const bool learned = learned_decisions(s, items[i].flags);
+        any_learned = any_learned || learned;
+        all_wide = all_wide && s->wide != nullptr && !(items[i].flags & RTQ_TRACE_BINARY) && ctx->prefer_wide &&
+                   (!learned || s->wide_ordered);
     }
     for (uint32_t i = 0; i < n_items; ++i) {
         const rtq_batch_item& it = items[i];
@@ -995,14 +1044,12 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
             return fail(ctx, RTQ_EINVAL, "trace_batch: all items must use the same leaf test");
         if (it.n_rays && (!it.rays_dev || !it.hits_dev))
             return fail(ctx, RTQ_EINVAL, "trace_batch: item %u: null buffer", i);
-        const int lay = pick_layout(ctx, s, it.flags, !any_ordered);
+        int lay = pick_layout(ctx, s, it.flags);   // refuses RTQ_TRACE_LEARNED without order bits
         if (lay < 0) return RTQ_ENOTBUILT;
-        const bool wide = lay == kWide;
-        // binary items with and without learned order share the ordered kernel
-        // (per-item order mask); the 8-wide layout cannot mix with them
-        if (layout < 0 || (layout == kBinary && lay == kOrdered)) layout = lay;
-        if (layout != lay && !(layout == kOrdered && lay == kBinary))
-            return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
+        if (any_learned) lay = all_wide ? kWideOrdered : kOrdered;
+        if (layout < 0) layout = lay;
+        if (layout != lay) return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
+        const bool wide = wide_layout(lay);
         BatchItem& b = table.item[i];
         b.nodes = wide ? s->wide : s->nodes;
         b.tris = s->tris;
@@ -1011,7 +1058,7 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
         b.begin = (uint32_t)total;
         b.n = (uint32_t)it.n_rays;
         b.any = (it.flags & RTQ_TRACE_ANY) ? 1u : 0u;
-        b.learned = lay == kOrdered ? 1u : 0u;
+        b.learned = learned_decisions(s, it.flags) ? 1u : 0u;
         total += it.n_rays;
         if (total > 0x80000000ull) return fail(ctx, RTQ_ECAPACITY, "trace_batch: more than 2^31 rays in one launch");
         depth = std::max(depth, stack_bound(s, wide));
@@ -1029,7 +1076,7 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
     if (one_scene) {
         args.nodes = table.item[0].nodes;
         args.tris = table.item[0].tris;
-        if (layout != kWide) {
+        if (!wide_layout(layout)) {
             args.top = s0->top;
             args.n_top = s0->n_top;
         }

@@ -7,10 +7,13 @@
 // contiguous in leaf order).  Next-level node indices come from a scan, so
 // numbering is deterministic and the children of a node are contiguous.
 //
-// Node (128 B, 8 float4; only the first 96 B are read by the traversal):
+// Node (128 B, 8 float4; the plain traversal reads the first 96 B):
 //   [ox oy oz meta] meta = ex | ey << 8 | ez << 16 | valid << 24
 //   [qlo_x[0..7] qhi_x[0..7]] [qlo_y.. qhi_y..] [qlo_z.. qhi_z..]   (bytes)
-//   [ref0..ref3] [ref4..ref7] [0 0 0 0] [0 0 0 0]
+//   [ref0..ref3] [ref4..ref7]
+//   [learned first child: 16 x 4 bits, lo hi 0 0]  (zero until an order model
+//       is applied; entry octant + 8 * any_hit = 8 | slot when decided)
+//   [0 0 0 0]
 // A child box is o + q * 2^(e - 127) per axis.  Boxes are widened by the scene
 // pad and quantised outward in exact (double) arithmetic, so the decoded box
 // always contains the padded child box: no hit is lost to quantisation
@@ -181,12 +184,15 @@ __device__ int exponent_for(double extent) {
 
 __global__ __launch_bounds__(kThreads) void k_wide_write(
     uint32_t n_items, uint32_t level_base, uint32_t next_base, uint32_t max_leaf, float pad,
-    const uint32_t* __restrict__ children, const uint32_t* __restrict__ n_children,
+    const uint32_t* __restrict__ items, const uint32_t* __restrict__ children, const uint32_t* __restrict__ n_children,
     const uint32_t* __restrict__ child_offs, const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ ccount,
-    const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ fl, float4* __restrict__ nodes,
+    const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ fl, const uint32_t* __restrict__ pos,
+    const uint32_t* __restrict__ bidx, float4* __restrict__ nodes, uint32_t* __restrict__ wide_trav,
     uint32_t* __restrict__ next_items) {
     const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= n_items) return;
+    // the binary traversal node of this node's root cluster (learned order)
+    wide_trav[level_base + k] = bidx ? bidx[pos[items[k]]] : kNone;
     const uint32_t n = n_children[k];
     float lo[kWidth][3], hi[kWidth][3];
     float org[3] = {INFINITY, INFINITY, INFINITY};
@@ -257,7 +263,8 @@ inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / 
 
 // Builds s->wide (s->n_wide nodes, s->wide_depth levels) from the cluster
 // arrays; fl = first leaf rank of every cluster (device).
-rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st) {
+rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, const uint32_t* pos, const uint32_t* bidx,
+                      hipStream_t st) {
     const uint32_t nc = s->n_clusters;
     const uint32_t root = nc - 1;
     const uint32_t cap = std::max<uint32_t>(s->n_trav, 1);  // wide nodes are a subset of the binary ones
@@ -273,6 +280,8 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st
     RTQ_HIP(ctx, tmp.get(&total, 1));
     // a refit writes the same buffer again (its size depends on the topology only)
     if (!s->wide) RTQ_HIP(ctx, dev_alloc(&s->wide, 8ull * cap));
+    if (!s->wide_trav) RTQ_HIP(ctx, dev_alloc(&s->wide_trav, cap));
+    s->wide_ordered = false;   // the nodes are rewritten with empty tables
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, items, root);
     RTQ_HIP(ctx, hipGetLastError());
     // cost table of the collapse, bottom-up over the PLOC passes
@@ -309,8 +318,8 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, hipStream_t st
         RTQ_HIP(ctx, hipStreamSynchronize(st));
         const uint32_t next_base = level_base + level_items;
         hipLaunchKernelGGL(k_wide_write, dim3(blocks_for(level_items)), dim3(kThreads), 0, st, level_items,
-                           level_base, next_base, s->params.max_leaf_size, s->box_pad, children, nchild, offs,
-                           s->cleft, s->ccount, s->cbox, fl, s->wide, next);
+                           level_base, next_base, s->params.max_leaf_size, s->box_pad, items, children, nchild, offs,
+                           s->cleft, s->ccount, s->cbox, fl, pos, bidx, s->wide, s->wide_trav, next);
         RTQ_HIP(ctx, hipGetLastError());
         std::swap(items, next);
         level_base = next_base;

@@ -49,6 +49,8 @@ def test_kernel_labels():
     assert bench.kernel_label("rtq::(anonymous namespace)::k_trace_occ<0, 2, false, 1, 2, 6>(x)") == \
         "k_trace<watertight,per-item,wide,batch>"
     assert bench.kernel_label("k_trace_occ<0, 0, false, 1, 0, 7>") == "k_trace<watertight,closest,wide>"
+    assert bench.kernel_label("k_trace_occ<0, 2, false, 3, 2, 6>") == "k_trace<watertight,per-item,learned_wide,batch>"
+    assert bench.kernel_label("k_trace_occ<1, 1, false, 2, 0, 7>") == "k_trace<refmt,any,learned>"
     assert bench.kernel_label("k_trace<0, 0, true, 0, 0>") is None   # counter instantiation
     assert bench.kernel_label("__amd_rocclr_copyBuffer") is None
 

@@ -299,6 +299,180 @@ def test_learned_order_exact(rtq_ctx, occ, torch_mod):
         trace(sc, occ["diff"], n, RTQ_TRACE_LEARNED)
 
 
+LEAF = 0x80000000
+
+
+def _leaf_range(r):
+    r = r.astype(np.int64)
+    first = (r & 0x7FFFFFFF) >> 3
+    return first, first + (r & 7) + 1
+
+
+def _node_ranges(refs, valid):
+    """[lo, hi) leaf-order triangle range of every node of a tree whose children
+    follow their parent: refs (n x w u32) with leaf refs or node indices."""
+    n, w = refs.shape
+    lo = np.full(n, 1 << 40, np.int64)
+    hi = np.full(n, -1, np.int64)
+    is_leaf = (refs & LEAF) != 0
+    llo, lhi = _leaf_range(refs)
+    for k in range(n - 1, -1, -1):
+        for j in range(w):
+            if not valid[k, j]:
+                continue
+            r = int(refs[k, j])
+            a, b = (int(llo[k, j]), int(lhi[k, j])) if is_leaf[k, j] else (lo[r], hi[r])
+            lo[k], hi[k] = min(lo[k], a), max(hi[k], b)
+    return lo, hi
+
+
+def wide_order_tables(trav, wide):
+    """The learned first-child table of every 8-wide node (words 24, 25), restated
+    from the binary nodes' order words: entry e = octant + 8 * any_hit follows the
+    confident binary decisions (bit 16 + e set; bit e set = child 0) from the
+    binary node covering the same triangles as the wide node until they reach
+    one of its children (a leaf child by its ref, an internal child by its
+    triangle range); 8 | slot there, 0 where a decision is missing."""
+    tu = trav.view(np.uint32)
+    brefs, order = tu[:, 12:14], tu[:, 14]
+    n_trav = len(tu)
+    blo, bhi = _node_ranges(brefs, np.ones((n_trav, 2), bool))
+    by_range = {(int(a), int(b)): x for x, (a, b) in enumerate(zip(blo, bhi))}
+    valid = ((wide[:, 3] >> 24)[:, None] >> np.arange(8)) & 1 != 0
+    wrefs = wide[:, 16:24]
+    wlo, whi = _node_ranges(wrefs, valid)
+    n_wide = len(wide)
+    # every child as the binary layout refers to it
+    as_bin = np.full((n_wide, 8), 0xFFFFFFFF, np.int64)
+    for k in range(n_wide):
+        for j in range(8):
+            if valid[k, j]:
+                r = int(wrefs[k, j])
+                as_bin[k, j] = r if r & LEAF else by_range.get((int(wlo[r]), int(whi[r])), -2)
+    root = np.array([by_range.get((int(a), int(b)), -1) for a, b in zip(wlo, whi)], np.int64)
+    tables = np.zeros((n_wide, 2), np.uint32)
+    for e in range(16):
+        x = root.copy()
+        live = x >= 0
+        entry = np.zeros(n_wide, np.uint32)
+        for _ in range(8):
+            xs = np.where(live, x, 0)
+            ow = order[xs]
+            live &= ((ow >> (16 + e)) & 1) != 0
+            child = np.where((ow >> e) & 1 != 0, brefs[xs, 0], brefs[xs, 1]).astype(np.int64)
+            match = as_bin == child[:, None]
+            found = live & match.any(axis=1)
+            slot = 7 - np.argmax(match[:, ::-1], axis=1)      # the last matching slot
+            entry[found] = 8 | slot[found]
+            live &= ~found & ((child & LEAF) == 0) & (child < n_trav)
+            x = np.where(live, child, x)
+        tables[:, e >> 3] |= entry << np.uint32(4 * (e & 7))
+    return tables
+
+
+def test_learned_wide_order(rtq_ctx, occ, torch_mod):
+    """Learned order on 8-wide nodes: the first-child tables against their
+    restatement from the binary order words, hit records against the plain
+    8-wide and binary kernels (trained, NaN and saturated weights), one launch
+    per set, batches (also with plain items, items of a binary-only scene, and
+    graph replays), and what clears the tables."""
+    torch = torch_mod
+    n = occ["n"]
+    wsc = rtq_ctx.upload(occ["tris"], occ["models"])
+    wsc.build(16, 4, 0, node_width=8)
+    try:
+        assert wsc.info().n_wide_nodes > 0
+        assert not wsc.export_wide()[:, 24:26].any()           # tables empty after a build
+        with pytest.raises(RtqError):
+            trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED)
+        plain_d = trace(wsc, occ["diff"], n, RTQ_TRACE_BINARY)
+        plain_s = trace(wsc, occ["shad"], 4 * n, RTQ_TRACE_ANY | RTQ_TRACE_BINARY)
+        assert hits_equal(trace(wsc, occ["diff"], n, 0), plain_d)
+        _, wide_v, wide_t = trace(wsc, occ["shad"], 4 * n, RTQ_TRACE_ANY, stats=True)
+        _, bin_v, _ = trace(wsc, occ["shad"], 4 * n, RTQ_TRACE_ANY | RTQ_TRACE_BINARY, stats=True)
+        model = OrderModel(rtq_ctx, seed=4)
+        samples, _ = model.log(wsc, occ["shad"], 4 * n, True, stride=3)
+        model.train(samples, epochs=3)
+        garbage = torch.full((ORDER_PARAMS,), float("nan"), device="cuda")
+        huge = dev(rand_params(5) * np.float32(1e30), torch)
+        for params, conf, label in ((model.params, 0.5, "trained"), (model.params, 0.8, "trained 0.8"),
+                                    (garbage, 0.5, "nan"), (huge, 0.5, "huge")):
+            torch.cuda.synchronize()
+            wsc.order_apply(params.data_ptr(), conf, conf)
+            trav, _ = wsc.export_trav()
+            wide = wsc.export_wide()
+            want = wide_order_tables(trav, wide)
+            assert np.array_equal(wide[:, 24:26], want), label
+            entries = (want[:, :, None] >> (4 * np.arange(8, dtype=np.uint32))) & 15
+            if label == "nan":
+                assert not entries.any()
+            else:
+                assert (entries & 8).mean() > 0.2, label
+            # a decided slot is a valid child
+            valid = (wide[:, 3] >> 24)[:, None, None] >> (entries & 7)
+            assert np.all((valid & 1)[(entries & 8) != 0] == 1)
+            got_d = trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED)
+            assert hits_equal(got_d, plain_d), f"{label}: {first_diff(got_d, plain_d)}"
+            got_s, lv, lt = trace(wsc, occ["shad"], 4 * n, RTQ_TRACE_ANY | RTQ_TRACE_LEARNED, stats=True)
+            assert np.array_equal(got_s["id"] >= 0, plain_s["id"] >= 0), label
+            # traced on the 8-wide nodes: about as many node visits as the plain
+            # 8-wide kernel, far fewer than on binary nodes
+            assert lv.sum() < 0.7 * bin_v.sum(), label
+            if label == "trained":
+                print(f"8-wide shadow work: nearest first {(wide_v + wide_t).sum()}, learned {(lv + lt).sum()} "
+                      f"({(lv + lt).sum() / (wide_v + wide_t).sum():.3f}x)")
+            # learned order on binary nodes, same records
+            assert hits_equal(trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED | RTQ_TRACE_BINARY), plain_d), label
+        wsc.order_apply(model.params.data_ptr(), 0.5, 0.5)
+        hd, hs = devmem.empty(n, HIT_DTYPE), devmem.empty(4 * n, HIT_DTYPE)
+        for flags_s in (RTQ_TRACE_ANY | RTQ_TRACE_LEARNED, RTQ_TRACE_ANY):   # learned + learned, learned + plain
+            items = rtq_ctx.items([(wsc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
+                                   (wsc, occ["shad"].data_ptr(), 4 * n, hs.data_ptr(), flags_s)])
+            rtq_ctx.trace_batch(items)
+            rtq_ctx.synchronize()
+            assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d), flags_s
+            assert np.array_equal(devmem.to_host(hs, HIT_DTYPE)["id"] >= 0, plain_s["id"] >= 0), flags_s
+        # graph replays, with eager work and a clear of the outputs between them
+        items = rtq_ctx.items([(wsc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
+                               (wsc, occ["shad"].data_ptr(), 4 * n, hs.data_ptr(), RTQ_TRACE_ANY | RTQ_TRACE_LEARNED)])
+        g = rtq_ctx.batch(items)
+        try:
+            for rep in range(3):
+                hd.fill_(255)
+                hs.fill_(255)
+                torch.cuda.synchronize()
+                g.launch()
+                rtq_ctx.synchronize()
+                assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d), rep
+                assert np.array_equal(devmem.to_host(hs, HIT_DTYPE)["id"] >= 0, plain_s["id"] >= 0), rep
+                assert hits_equal(trace(wsc, occ["diff"], 1000, 0), plain_d[:1000])
+        finally:
+            g.destroy()
+        # with an item of a binary-only scene the batch traces binary nodes
+        # with the learned order (occ's scene holds no order bits here)
+        sc = occ["sc"]
+        sc.order_apply(None)
+        h2 = devmem.empty(n, HIT_DTYPE)
+        both = rtq_ctx.items([(wsc, occ["diff"].data_ptr(), n, hd.data_ptr(), RTQ_TRACE_LEARNED),
+                              (sc, occ["diff"].data_ptr(), n, h2.data_ptr(), 0)])
+        rtq_ctx.trace_batch(both)
+        rtq_ctx.synchronize()
+        assert hits_equal(devmem.to_host(hd, HIT_DTYPE), plain_d)
+        assert hits_equal(devmem.to_host(h2, HIT_DTYPE), plain_d)
+        # clearing the order empties the tables; a rebuild does as well
+        wsc.order_apply(None)
+        assert not wsc.export_wide()[:, 24:26].any()
+        with pytest.raises(RtqError):
+            trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED)
+        wsc.order_apply(model.params.data_ptr(), 0.5, 0.5)
+        wsc.build(16, 4, 0, node_width=8)
+        assert not wsc.export_wide()[:, 24:26].any()
+        with pytest.raises(RtqError):
+            trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED)
+    finally:
+        wsc.release()
+
+
 def test_learned_order_gain(rtq_ctx, occ, torch_mod):
     """A model trained on the shadow-ray logs of one light set needs fewer node
     visits + triangle tests than the nearer-child order on the shadow rays of

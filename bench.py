@@ -421,7 +421,8 @@ def main() -> int:
     learned_setup = None
     for form in args.forms.split(","):
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
         tuned = None
         if form in ("plain", "compressed"):
             layout = RTQ_TRACE_BINARY if form == "plain" else 0
@@ -447,9 +448,10 @@ def main() -> int:
             if form == "batched" and args.batched_layout != "tuned":
                 tr.best.prefer_wide = int(args.batched_layout == "wide")
                 ctx.set_launch_params(tr.best.refill, tr.best.chunk, bool(tr.best.prefer_wide), tr.best.spec_slack,
-                                      tr.best.leaf_slack)
+                                      tr.best.leaf_slack, tr.best.stack_lds)
             tuned = {"refill": tr.best.refill, "chunk": tr.best.chunk, "prefer_wide": tr.best.prefer_wide,
-                     "spec_slack": tr.best.spec_slack, "leaf_slack": tr.best.leaf_slack, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
+                     "spec_slack": tr.best.spec_slack, "leaf_slack": tr.best.leaf_slack,
+                     "stack_lds": tr.best.stack_lds, "candidates": tr.n_tried, "best_ms": round(tr.best_ms, 4),
                      "default_ms": round(tr.default_ms, 4)}
             if form == "learned" and wl.sc.info().order_modes:
                 # learned decisions: the batch traces 8-wide nodes with their
@@ -483,7 +485,8 @@ def main() -> int:
             "roofline": roofline_of(profile, kernel, dom_ms, dom_bytes, node_bytes),
         }
     ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
 
     best = max(results, key=lambda f: results[f]["Mrays_s"])
     res = results[best]
@@ -613,7 +616,8 @@ def main() -> int:
         same = pf == ps and all(np.array_equal(cf[k], cs[k]) for k in ("triangle_id", "left", "right", "parent")) \
             and np.array_equal(cf["boxes"].view(np.uint8), cs["boxes"].view(np.uint8))
         build = {"workload": "terrain 1024x1024 (2,093,058 tris): bounds, Morton codes, sort, PLOC (radius 16), "
-                             "reference node array, binary and 8-wide traversal layouts; best of 3",
+                             "pre-order placement, leaf-ordered triangles, binary and 8-wide traversal layouts "
+                             "(the reference node array is written on export, outside this time); best of 3",
                  "fused_ms": round(times["fused"], 3), "split_ms": round(times["split"], 3),
                  "fused_vs_split": round(times["split"] / times["fused"], 3), "ploc_passes": pf,
                  "same_clusters": bool(same)}
@@ -673,7 +677,8 @@ def main() -> int:
         ocam = scenes.pinhole(**{**scenes.TERRAIN_CAMERA, "width": args.occluded_res,
                                  "height": args.occluded_res})
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
         owl = Workload(ctx, otris, omodels, ocam, sp, args.max_leaf, node_width=8)
         n_otris = len(otris)
         del otris
@@ -688,15 +693,20 @@ def main() -> int:
                 flags = RTQ_TRACE_LEARNED | (RTQ_TRACE_BINARY if form == "learned_binary" else 0)
             if form == "learned_batched":
                 ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
                 items = ctx.items([(owl.sc, owl.diff.data_ptr(), owl.n_diff, owl.hd.data_ptr(), flags),
                                    (owl.sc, owl.shad.data_ptr(), owl.n_shad, owl.hs.data_ptr(),
                                     RTQ_TRACE_ANY | flags)])
-                ctx.autotune(items, reps=2)
+                otr = ctx.autotune(items, reps=2)
+                otuned = {"refill": otr.best.refill, "chunk": otr.best.chunk, "prefer_wide": otr.best.prefer_wide,
+                          "spec_slack": otr.best.spec_slack, "leaf_slack": otr.best.leaf_slack,
+                          "stack_lds": otr.best.stack_lds, "candidates": otr.n_tried}
                 graph = ctx.batch(items)
                 ms_per_step, evs = time_steps(graph_step(graph))
                 graph.destroy()
             else:
+                otuned = None
                 ms_per_step, evs = time_steps(two_launches(owl, flags))
             got = owl.hits()
             if oref is None:
@@ -711,7 +721,7 @@ def main() -> int:
                 "shadow_ms": round(float(np.mean([e[1].elapsed_time(e[2]) for e in evs])), 4) if split else None,
                 "shadow_node_visits": c["shadow"]["node_visits"], "shadow_tri_tests": c["shadow"]["tri_tests"],
                 "occluded_fraction": round(float((got[1]["id"] >= 0).mean()), 4),
-                "exact": same_results(oref, got), "order_model": setup}
+                "exact": same_results(oref, got), "order_model": setup, "tuned": otuned}
         occluded = {
             "workload": f"terrain 1024x1024 + 2,097,152 soup triangles ({n_otris:,} tris), "
                         f"{args.occluded_res}x{args.occluded_res} primary hits -> 1 diffuse + 4 shadow rays each",
@@ -732,7 +742,8 @@ def main() -> int:
         t4 = time.time()
         # an earlier block's autotune may have left binary nodes preferred
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
         ttris = scenes.tiled_terrains(1024, 2, seed=0)
         tsc = ctx.upload(ttris, scenes.identity_models())
         tinfo = tsc.build(16, args.max_leaf, 0, node_width=8)
@@ -785,7 +796,8 @@ def main() -> int:
         from realtimeraytracing_amd import HIT_DTYPE, RAY_DTYPE
         t5 = time.time()
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
         bsc, brays, bsep, bbat, bn = [], [], [], [], []
         n_tris_total = 0
         for s_ in range(64):
@@ -827,7 +839,7 @@ def main() -> int:
             "batched_graph": {"Mrays_s": round(total_rays / (bat_ms * 1e-3) / 1e6, 2), "ms": round(bat_ms, 4),
                               "tuned": {"refill": tr.best.refill, "chunk": tr.best.chunk,
                                         "prefer_wide": tr.best.prefer_wide, "spec_slack": tr.best.spec_slack,
-                                        "leaf_slack": tr.best.leaf_slack,
+                                        "leaf_slack": tr.best.leaf_slack, "stack_lds": tr.best.stack_lds,
                                         "candidates": tr.n_tried}},
             "batched_vs_separate": round(sep_ms / bat_ms, 3), "exact": exact,
             "generation_s": round(gen_s, 1), "setup_s": round(time.time() - t5, 1)}
@@ -835,7 +847,8 @@ def main() -> int:
             sc_.release()
         del brays, bsep, bbat
         ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
-                              default_params.spec_slack, default_params.leaf_slack)
+                              default_params.spec_slack, default_params.leaf_slack,
+                              default_params.stack_lds)
 
     def rounded(d):
         return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}

@@ -315,11 +315,15 @@ typedef struct rtq_launch_params {
     uint32_t leaf_slack;   /* ... and back to node visits once at most this many (0..63)
                               lanes still have triangles to test; those lanes resume
                               their leaf in the next leaf phase (0: none) */
-    uint32_t reserved;     /* 0 */
+    uint32_t stack_lds;    /* traversal stack entries per ray held in LDS by batched
+                              traces (rtq_trace_batch, rtq_batch_*), deeper entries in
+                              HBM: 0 = the layout's default; binary layouts take 8, 12
+                              (default) or 14, 8-wide layouts 8, 16 (default) or 20, and
+                              a value of the other family means the default */
 } rtq_launch_params;
 
 /* A new context starts with refill 32, chunk 128, prefer_wide 1, spec_slack 16,
- * leaf_slack 8. */
+ * leaf_slack 8, stack_lds 0. */
 rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out);
 rtq_status rtq_launch_params_set(rtq_ctx* ctx, const rtq_launch_params* p);
 
@@ -332,7 +336,7 @@ typedef struct rtq_tune_result {
 } rtq_tune_result;
 
 /* Times candidate launch settings (refill, chunk, node layout, then the
- * speculation and leaf slacks) on the given batch, `reps` launches each, times
+ * speculation and leaf slacks, then the LDS stack depth) on the given batch, `reps` launches each, times
  * the four fastest again in three rounds, applies the fastest to the context and records
  * it under a key of (device, scene sizes, ray counts, modes); a later call
  * with the same key reuses it without timing.  Writes the items' hit buffers.

@@ -41,7 +41,7 @@ uint64_t tune_key(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n) {
 
 bool valid_params(const rtq_launch_params& p) {
     return p.refill >= 1 && p.refill <= 64 && p.chunk >= 1 && p.chunk <= 4096 && p.prefer_wide <= 1 &&
-           p.spec_slack <= 63 && p.leaf_slack <= 63 && p.reserved == 0;
+           p.spec_slack <= 63 && p.leaf_slack <= 63 && valid_stack_lds(p.stack_lds);
 }
 
 void apply(rtq_ctx* ctx, const rtq_launch_params& p) {
@@ -50,6 +50,7 @@ void apply(rtq_ctx* ctx, const rtq_launch_params& p) {
     ctx->prefer_wide = p.prefer_wide != 0;
     ctx->spec_slack = p.spec_slack;
     ctx->leaf_slack = p.leaf_slack;
+    ctx->stack_lds = p.stack_lds;
 }
 
 rtq_launch_params current(const rtq_ctx* ctx) {
@@ -59,6 +60,7 @@ rtq_launch_params current(const rtq_ctx* ctx) {
     p.prefer_wide = ctx->prefer_wide ? 1u : 0u;
     p.spec_slack = ctx->spec_slack;
     p.leaf_slack = ctx->leaf_slack;
+    p.stack_lds = ctx->stack_lds;
     return p;
 }
 
@@ -152,7 +154,10 @@ rtq_status rtq_launch_params_get(rtq_ctx* ctx, rtq_launch_params* out) {
 
 rtq_status rtq_launch_params_set(rtq_ctx* ctx, const rtq_launch_params* p) {
     if (!ctx || !p) return RTQ_EINVAL;
-    if (!valid_params(*p)) return fail(ctx, RTQ_EINVAL, "launch_params: refill 1..64, chunk 1..4096, prefer_wide 0|1, slacks 0..63, reserved 0");
+    if (!valid_params(*p))
+        return fail(ctx, RTQ_EINVAL,
+                    "launch_params: refill 1..64, chunk 1..4096, prefer_wide 0|1, slacks 0..63, stack_lds 0 or one of "
+                    "8, 12, 14 (binary), 8, 16, 20 (8-wide)");
     apply(ctx, *p);
     return RTQ_OK;
 }
@@ -223,15 +228,38 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
         for (uint32_t rf : refills)
             for (uint32_t ch : chunks)
                 if (status == RTQ_OK)
-                    status = consider(rtq_launch_params{rf, ch, wide, before.spec_slack, before.leaf_slack, 0});
+                    status = consider(rtq_launch_params{rf, ch, wide, before.spec_slack, before.leaf_slack,
+                                                        before.stack_lds});
     const rtq_launch_params stage1 = best;
     const uint32_t spec[] = {0, 4, 8, 16, 24, 32};
     const uint32_t leaf[] = {0, 8, 16, 24};
     for (uint32_t sl : spec)
         for (uint32_t ll : leaf)
             if (status == RTQ_OK && !(sl == stage1.spec_slack && ll == stage1.leaf_slack))
-                status = consider(rtq_launch_params{stage1.refill, stage1.chunk, stage1.prefer_wide, sl, ll, 0});
-    // stage 3: one timing of a ~1 ms launch spreads by a few percent, enough
+                status = consider(rtq_launch_params{stage1.refill, stage1.chunk, stage1.prefer_wide, sl, ll,
+                                                    stage1.stack_lds});
+    // stage 3: the LDS stack depths of the layout the best settings trace
+    // (stored explicitly, so the result names the depth)
+    if (status == RTQ_OK) {
+        const rtq_launch_params stage2 = best;
+        apply(ctx, stage2);
+        int layout = -1;
+        status = batch_layout(ctx, items, n_items, &layout);
+        const uint32_t* choices = (layout == 1 || layout == 3) ? kStackChoicesWide : kStackChoicesBinary;
+        for (uint32_t k = 0; k < 3 && status == RTQ_OK; ++k) {
+            rtq_launch_params p = stage2;
+            p.stack_lds = choices[k];
+            if (k == 1 && (stage2.stack_lds == 0 || stage2.stack_lds == choices[1])) {
+                // the depth in force: relabel its timings instead of timing it again
+                for (auto& sp : seen)
+                    if (memcmp(&sp.second, &stage2, sizeof stage2) == 0) sp.second.stack_lds = choices[1];
+                if (memcmp(&best, &stage2, sizeof best) == 0) best.stack_lds = choices[1];
+                continue;
+            }
+            if (p.stack_lds != stage2.stack_lds) status = consider(p);
+        }
+    }
+    // stage 4: one timing of a ~1 ms launch spreads by a few percent, enough
     // to misorder close settings, so the four fastest run again in three
     // interleaved rounds and each keeps its lowest time
     if (status == RTQ_OK) {
@@ -250,6 +278,13 @@ rtq_status rtq_autotune(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_it
             best = seen[w].second;
             best_ms = low[w];
         }
+    }
+    if (status == RTQ_OK && best.stack_lds == 0) {
+        // name the default depth of the layout the chosen settings trace
+        apply(ctx, best);
+        int layout = -1;
+        status = batch_layout(ctx, items, n_items, &layout);
+        best.stack_lds = (layout == 1 || layout == 3) ? kStackChoicesWide[1] : kStackChoicesBinary[1];
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

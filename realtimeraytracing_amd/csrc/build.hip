@@ -865,11 +865,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_place_small(const uint2* __re
     }
 }
 
-// Leaf-ordered traversal triangles and the depth of the deepest leaf.
+// Leaf-ordered traversal triangles, and the depth of each workgroup's deepest
+// leaf in block_depth[blockIdx.x] (k_max_u32 takes the maximum: atomics of
+// every wave on one word would serialise, tens of thousands of them).
 __global__ __launch_bounds__(kThreads) void k_write_leaves(
     uint32_t n_clusters, const uint32_t* __restrict__ ctri, const uint32_t* __restrict__ cleft,
     const uint32_t* __restrict__ fl, const uint32_t* __restrict__ dep, const float4* __restrict__ tri_world,
-    float4* __restrict__ tris, uint32_t* __restrict__ max_depth) {
+    float4* __restrict__ tris, uint32_t* __restrict__ block_depth) {
+    __shared__ uint32_t s_max[kThreads / 64];
     const uint32_t c = blockIdx.x * kThreads + threadIdx.x;
     uint32_t depth = 0;
     if (c < n_clusters && cleft[c] == kNone) {
@@ -880,10 +883,31 @@ __global__ __launch_bounds__(kThreads) void k_write_leaves(
         tris[dst + 2] = tri_world[3ull * t + 2];
         depth = dep[c];
     }
-    // deepest leaf: one atomic per wave instead of one per leaf
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) depth = max(depth, (uint32_t)__shfl_xor(depth, d, 64));
-    if ((threadIdx.x & 63) == 0 && depth > 0) atomicMax(max_depth, depth);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x / 64] = depth;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (uint32_t w = 0; w < kThreads / 64; ++w) m = max(m, s_max[w]);
+        block_depth[blockIdx.x] = m;
+    }
+}
+
+// out[0] = max of v[0..n) (one workgroup of 1024 lanes).
+__global__ __launch_bounds__(1024) void k_max_u32(const uint32_t* __restrict__ v, uint32_t n,
+                                                  uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_max[16];
+    uint32_t m = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) m = max(m, v[i]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < 16; ++w) m = max(m, s_max[w]);
+        out[0] = m;
+    }
 }
 
 // Reference node array (scene.cpp:189-208): cluster c at its pre-order
@@ -1153,10 +1177,11 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     RTQ_HIP(ctx, tmp.get(&fl, nc));
     RTQ_HIP(ctx, tmp.get(&dep, nc));
     RTQ_HIP(ctx, tmp.get(&maxd, 2));
+    uint32_t* bdep = nullptr;
+    RTQ_HIP(ctx, tmp.get(&bdep, blocks_for(nc)));
     RTQ_HIP(ctx, tmp.get(&flag, nc));
     RTQ_HIP(ctx, tmp.get(&bidx, nc));
     RTQ_HIP(ctx, tmp.get(&ntrav, 1));
-    RTQ_HIP(ctx, hipMemsetAsync(maxd, 0, sizeof(uint32_t) * 2, st));
     const uint32_t root = nc - 1;
     hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
     RTQ_HIP(ctx, hipGetLastError());
@@ -1175,7 +1200,9 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
         RTQ_HIP(ctx, hipGetLastError());
     }
     hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->ctri, s->cleft, fl,
-                       dep, s->tri_world, s->tris, maxd);
+                       dep, s->tri_world, s->tris, bdep);
+    RTQ_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, bdep, blocks_for(nc), maxd);
     RTQ_HIP(ctx, hipGetLastError());
 
     rtq_aabb h_bounds[2];
@@ -1255,9 +1282,10 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     // triangle: sort keys and values and the PLOC lists 64 B, placement 32 B;
     // with 8-wide nodes also the collapse cost table 36 B and the collapse
     // buffers 52 B per binary traversal node (one per triangle with leaves of
-    // one triangle); 1 MiB for scans and 256-B rounding
+    // one triangle); each placement workgroup's deepest leaf n / 32 B; 1 MiB
+    // for scans and 256-B rounding
     const size_t per_tri = prm.node_width == 8 ? 64 + 32 + 36 + 52 : 64 + 32;
-    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + ((size_t)1 << 20));
+    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + n / 32 + ((size_t)1 << 20));
     rtq_status rs = arena_reserve(ctx);
     if (rs != RTQ_OK) return rs;
 

@@ -24,6 +24,27 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kMaxTris = (1u << 28) - 16u;
 constexpr uint32_t kMaxLeaf = 8u;
 
+// Traversal stack entries per lane held in LDS (deeper entries spill to HBM):
+// binary layouts share the workgroup's LDS with the top-of-tree copy, the
+// 8-wide layout (up to 7 pushes per level, no top copy) keeps a deeper one.
+// Batched launches take one of three depths per layout family (the default in
+// the middle); the deepest keeps 6 (8-wide) or 7 (binary) workgroups of 256
+// lanes within a CU's 160 KB of LDS next to the batch item table.
+#ifndef RTQ_STACK_LDS
+#define RTQ_STACK_LDS 12
+#endif
+#ifndef RTQ_STACK_LDS_WIDE
+#define RTQ_STACK_LDS_WIDE 16
+#endif
+constexpr uint32_t kStackChoicesBinary[3] = {8, RTQ_STACK_LDS, 14};
+constexpr uint32_t kStackChoicesWide[3] = {8, RTQ_STACK_LDS_WIDE, 20};
+inline bool valid_stack_lds(uint32_t v) {
+    if (v == 0) return true;
+    for (uint32_t k = 0; k < 3; ++k)
+        if (v == kStackChoicesBinary[k] || v == kStackChoicesWide[k]) return true;
+    return false;
+}
+
 // Top of the binary traversal tree, staged in LDS by every trace workgroup:
 // the first kTopNodes internal nodes in BFS order from the root, copied with
 // the child refs that stay inside this set rewritten to kTopBit | slot
@@ -151,6 +172,8 @@ struct rtq_ctx {
     // node visits resume once at most leaf_slack lanes still test triangles
     uint32_t leaf_slack = 8;
     bool prefer_wide = true;
+    // LDS stack entries per lane of batched launches (0: the layout's default)
+    uint32_t stack_lds = 0;
     // autotuner results: key -> launch settings
     std::map<uint64_t, rtq_launch_params> tuned;
 };
@@ -245,6 +268,10 @@ rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, 
                  uint32_t* stat_nodes, uint32_t* stat_tris, bool stats, hipStream_t st);
 rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
                        bool size_only, hipStream_t st);
+// Node layout a batch traces under the context's settings (checks the items):
+// 0 binary, 1 8-wide, 2 binary with learned order, 3 8-wide with learned
+// first children.
+rtq_status batch_layout(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, int* layout);
 rtq_status scratch_alloc(rtq_ctx* ctx, TraceScratch* sc);
 // rtq_trace_noalloc (trace.hip): per-device address of the fixed scratch
 // array, looked up once per device at context creation (device current).

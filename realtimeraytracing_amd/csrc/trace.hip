@@ -57,23 +57,19 @@ constexpr uint32_t kDry = kMaxParts * kPartStride;
 constexpr uint32_t kArrive = kDry + kMaxParts / 32;
 constexpr uint32_t kCounterWords = kArrive + kPartStride;
 
-// Traversal stack entries per lane held in LDS (deeper entries spill to HBM):
-// binary layouts share the workgroup's LDS with the top-of-tree copy, the
-// 8-wide layout (up to 7 pushes per level, no top copy) keeps a deeper one.
-#ifndef RTQ_STACK_LDS
-#define RTQ_STACK_LDS 12
-#endif
-#ifndef RTQ_STACK_LDS_WIDE
-#define RTQ_STACK_LDS_WIDE 16
-#endif
 // Relative widening of a box's far distance (tn <= tf + |tf| * kSlabRel):
 // 32 units of 2^-24, twice the sum of the distance-proportional errors of the
 // box test (reciprocal 2, o * inv 1, FMA 1, 8-wide node base 1) and of the
 // watertight test (vertex translation 1, t = T / det about 6), plus the
 // rounding of the widening itself.
 constexpr float kSlabRel = 0x1p-19f;
-constexpr uint32_t kStackLdsBinary = RTQ_STACK_LDS;
-constexpr uint32_t kStackLdsWide = RTQ_STACK_LDS_WIDE;
+// Newest stack entries held in registers (0: none) before LDS: a pop then
+// hands over its node at once, and the register refill from LDS completes
+// behind the next node fetch.
+#ifndef RTQ_STACK_REG
+#define RTQ_STACK_REG 0
+#endif
+constexpr int kStackReg = RTQ_STACK_REG;
 // Optional VGPR budget of the trace kernels (occupancy is 512 / budget waves
 // per SIMD), or an occupancy the compiler must fit (waves per SIMD).
 #ifdef RTQ_TRACE_VGPRS
@@ -259,9 +255,9 @@ __host__ __device__ constexpr bool learned_layout(int layout) { return layout ==
 // writes the hit.
 // The body of the trace kernels below (k_trace, and k_trace_occ with an
 // occupancy hint).
-template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int STACK>
 __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<(MULTI != 0)>& table) {
-    constexpr uint32_t kStackLds = wide_layout(LAYOUT) ? kStackLdsWide : kStackLdsBinary;
+    constexpr uint32_t kStackLds = STACK;
     __shared__ uint32_t s_stack[kStackLds * kTraceThreads];
     __shared__ BatchItem s_items[MULTI ? kMaxBatchItems : 1];
     // the top of the binary tree, read from LDS instead of L2 by every ray
@@ -313,21 +309,47 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
     uint32_t leaf_k = 0;  // next triangle of `leaf`
     uint32_t n_nodes = 0, n_tests = 0;
 
-    auto push = [&](uint32_t v) {
-        if (sp < kStackLds) {
-            lds[sp * kTraceThreads] = v;
+    // entry k of the stack's memory part: LDS, then the HBM spill column
+    auto mem_store = [&](uint32_t k, uint32_t v) {
+        if (k < kStackLds) {
+            lds[k * kTraceThreads] = v;
         } else {
-            __hip_atomic_store(spill + (size_t)(sp - kStackLds) * stride, v, __ATOMIC_RELAXED,
+            __hip_atomic_store(spill + (size_t)(k - kStackLds) * stride, v, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    };
+    auto mem_load = [&](uint32_t k) -> uint32_t {
+        if (k < kStackLds) return lds[k * kTraceThreads];
+        return __hip_atomic_load(spill + (size_t)(k - kStackLds) * stride, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    // sp entries in all; with a register top the newest min(sp, kStackReg)
+    // are reg[0] (newest) .. reg[kStackReg - 1], the older ones memory
+    // entries 0 .. sp - kStackReg - 1
+    uint32_t reg[kStackReg > 0 ? kStackReg : 1] = {};
+    auto push = [&](uint32_t v) {
+        if constexpr (kStackReg > 0) {
+            if (sp >= (uint32_t)kStackReg) mem_store(sp - kStackReg, reg[kStackReg - 1]);
+#pragma unroll
+            for (int i = kStackReg - 1; i > 0; --i) reg[i] = reg[i - 1];
+            reg[0] = v;
+        } else {
+            mem_store(sp, v);
         }
         ++sp;
     };
     auto pop = [&]() -> uint32_t {
         if (sp == 0) return kEmpty;
         --sp;
-        if (sp < kStackLds) return lds[sp * kTraceThreads];
-        return __hip_atomic_load(spill + (size_t)(sp - kStackLds) * stride, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        if constexpr (kStackReg > 0) {
+            const uint32_t v = reg[0];
+#pragma unroll
+            for (int i = 0; i < kStackReg - 1; ++i) reg[i] = reg[i + 1];
+            if (sp >= (uint32_t)kStackReg) reg[kStackReg - 1] = mem_load(sp - kStackReg);
+            return v;
+        } else {
+            return mem_load(sp);
+        }
     };
     auto finish = [&]() {
         rtq_hit h;
@@ -704,18 +726,18 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
     }
 }
 
-template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int STACK>
 __global__ __launch_bounds__(kTraceThreads) RTQ_TRACE_OCCUPANCY void k_trace(TraceArgs a,
                                                                             ItemTable<(MULTI != 0)> table) {
-    trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI>(a, table);
+    trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI, STACK>(a, table);
 }
 
 // The same kernel with an occupancy (waves per SIMD) the register allocation
 // must fit.
-template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int WAVES>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int WAVES, int STACK>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace_occ(
     TraceArgs a, ItemTable<(MULTI != 0)> table) {
-    trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI>(a, table);
+    trace_waves<LEAF, MODE, STATS, LAYOUT, MULTI, STACK>(a, table);
 }
 
 // Occupancy hints by layout (0 = the compiler's own choice), from timings on
@@ -754,28 +776,34 @@ constexpr int occupancy_hint() {
     return RTQ_OCC_BATCH;  // binary, with or without the learned order
 }
 
-template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int STACK>
 auto trace_kernel() {
     constexpr int w = occupancy_hint<LAYOUT, MULTI>();
     if constexpr (w > 0)
-        return &k_trace_occ<LEAF, MODE, STATS, LAYOUT, MULTI, w>;
+        return &k_trace_occ<LEAF, MODE, STATS, LAYOUT, MULTI, w, STACK>;
     else
-        return &k_trace<LEAF, MODE, STATS, LAYOUT, MULTI>;
+        return &k_trace<LEAF, MODE, STATS, LAYOUT, MULTI, STACK>;
+}
+
+// LDS stack entries per lane: choice 0, 1 (the default) or 2 of the layout's family.
+template <int LAYOUT>
+constexpr int stack_choice(int k) {
+    return (int)(wide_layout(LAYOUT) ? kStackChoicesWide[k] : kStackChoicesBinary[k]);
 }
 
 // Persistent grid sized from occupancy, spill column sized from the stack
 // bound `depth_entries`, one launch (or only the scratch sizing).
-template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI>
+template <int LEAF, int MODE, bool STATS, int LAYOUT, int MULTI, int STACK = stack_choice<LAYOUT>(1)>
 rtq_status launch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth_entries, const TraceArgs& base,
                   const ItemTable<(MULTI != 0)>& table, uint32_t n, bool size_only, hipStream_t st) {
-    const auto kern = trace_kernel<LEAF, MODE, STATS, LAYOUT, MULTI>();
+    const auto kern = trace_kernel<LEAF, MODE, STATS, LAYOUT, MULTI, STACK>();
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTraceThreads, 0);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     uint64_t blocks = std::min<uint64_t>((n + kTraceThreads - 1) / kTraceThreads,
                                          (uint64_t)per_cu * (uint64_t)std::max(ctx->n_cu, 1));
     if (blocks == 0) blocks = 1;
-    constexpr uint32_t kStackLds = wide_layout(LAYOUT) ? kStackLdsWide : kStackLdsBinary;
+    constexpr uint32_t kStackLds = STACK;
     const uint64_t column = depth_entries > kStackLds ? depth_entries - kStackLds : 0;  // spill words per thread
     if (sc.fixed && column > 0) {
         // a spill buffer that cannot grow: as many workgroups as it holds
@@ -987,36 +1015,47 @@ rtq_status trace(rtq_ctx* ctx, const Scene* s, const rtq_ray* rays, uint64_t n, 
                : launch_single<1, 0, true>(ctx, s, args, nn, layout, st);
 }
 
+// A batched launch at the context's LDS stack depth (stack_lds 0 or a depth
+// of the other layout family: the default).
+template <int LEAF, int LAYOUT, int MULTI>
+rtq_status launch_batch_stack(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth, const TraceArgs& args,
+                              const ItemTable<true>& table, uint32_t nt, bool size_only, hipStream_t st) {
+    const int want = (int)ctx->stack_lds;
+    if (want == stack_choice<LAYOUT>(0))
+        return launch<LEAF, 2, false, LAYOUT, MULTI, stack_choice<LAYOUT>(0)>(ctx, sc, depth, args, table, nt,
+                                                                             size_only, st);
+    if (want == stack_choice<LAYOUT>(2))
+        return launch<LEAF, 2, false, LAYOUT, MULTI, stack_choice<LAYOUT>(2)>(ctx, sc, depth, args, table, nt,
+                                                                             size_only, st);
+    return launch<LEAF, 2, false, LAYOUT, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+}
+
 template <int MULTI>
 rtq_status launch_batch(rtq_ctx* ctx, TraceScratch& sc, uint64_t depth, const TraceArgs& args,
                         const ItemTable<true>& table, uint32_t nt, bool size_only, hipStream_t st, bool refmt,
                         int layout) {
     if (refmt) {
-        if (layout == kWide) return launch<1, 2, false, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+        if (layout == kWide) return launch_batch_stack<1, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
         if (layout == kWideOrdered)
-            return launch<1, 2, false, kWideOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+            return launch_batch_stack<1, kWideOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
         if (layout == kOrdered)
-            return launch<1, 2, false, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
-        return launch<1, 2, false, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+            return launch_batch_stack<1, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+        return launch_batch_stack<1, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
     }
-    if (layout == kWide) return launch<0, 2, false, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (layout == kWide) return launch_batch_stack<0, kWide, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
     if (layout == kWideOrdered)
-        return launch<0, 2, false, kWideOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
-    if (layout == kOrdered) return launch<0, 2, false, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
-    return launch<0, 2, false, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+        return launch_batch_stack<0, kWideOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    if (layout == kOrdered)
+        return launch_batch_stack<0, kOrdered, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
+    return launch_batch_stack<0, kBinary, MULTI>(ctx, sc, depth, args, table, nt, size_only, st);
 }
 
-// Up to kMaxBatchItems ray sets (any scenes, same leaf test and layout) in one
-// persistent launch: waves take chunks of the concatenated ray index space
-// (their XCD's part first, then any other part), so uneven sets balance across
-// the whole chip.
-rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
-                       bool size_only, hipStream_t st) {
+// Checks a batch's items and picks the node layout they trace together.
+rtq_status batch_layout(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, int* out) {
+    *out = -1;
     if (n_items == 0) return RTQ_OK;
     if (!items) return fail(ctx, RTQ_EINVAL, "trace_batch: null item array");
     if (n_items > kMaxBatchItems) return fail(ctx, RTQ_ECAPACITY, "trace_batch: at most %u items per launch", kMaxBatchItems);
-    ItemTable<true> table{};
-    uint64_t total = 0, depth = 0;
     const uint32_t leaf_flags = items[0].flags & RTQ_TRACE_LEAF_REFMT;
     int layout = -1;
     // Items with learned decisions trace a learned layout, and so does the
@@ -1049,7 +1088,27 @@ rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_ite
         if (any_learned) lay = all_wide ? kWideOrdered : kOrdered;
         if (layout < 0) layout = lay;
         if (layout != lay) return fail(ctx, RTQ_EINVAL, "trace_batch: all items must trace the same node layout");
-        const bool wide = wide_layout(lay);
+    }
+    *out = layout;
+    return RTQ_OK;
+}
+
+// Up to kMaxBatchItems ray sets (any scenes, same leaf test and layout) in one
+// persistent launch: waves take chunks of the concatenated ray index space
+// (their XCD's part first, then any other part), so uneven sets balance across
+// the whole chip.
+rtq_status trace_batch(rtq_ctx* ctx, const rtq_batch_item* items, uint32_t n_items, TraceScratch* scratch,
+                       bool size_only, hipStream_t st) {
+    int layout = -1;
+    const rtq_status rs = batch_layout(ctx, items, n_items, &layout);
+    if (rs != RTQ_OK || n_items == 0) return rs;
+    ItemTable<true> table{};
+    uint64_t total = 0, depth = 0;
+    const uint32_t leaf_flags = items[0].flags & RTQ_TRACE_LEAF_REFMT;
+    const bool wide = wide_layout(layout);
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const rtq_batch_item& it = items[i];
+        const Scene* s = find_scene(ctx, it.scene_id);
         BatchItem& b = table.item[i];
         b.nodes = wide ? s->wide : s->nodes;
         b.tris = s->tris;

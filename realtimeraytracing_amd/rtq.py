@@ -69,7 +69,7 @@ class BatchItem(ctypes.Structure):
 
 class LaunchParams(ctypes.Structure):
     _fields_ = [("refill", ctypes.c_uint32), ("chunk", ctypes.c_uint32), ("prefer_wide", ctypes.c_uint32),
-                ("spec_slack", ctypes.c_uint32), ("leaf_slack", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("spec_slack", ctypes.c_uint32), ("leaf_slack", ctypes.c_uint32), ("stack_lds", ctypes.c_uint32)]
 
 
 class TuneResult(ctypes.Structure):
@@ -291,8 +291,9 @@ class Context:
         return p
 
     def set_launch_params(self, refill: int, chunk: int, prefer_wide: bool = True, spec_slack: int = 16,
-                          leaf_slack: int = 8) -> None:
-        p = LaunchParams(refill, chunk, int(prefer_wide), spec_slack, leaf_slack, 0)
+                          leaf_slack: int = 8, stack_lds: int = 0) -> None:
+        """stack_lds: LDS stack entries of batched traces, 0 = the layout's default."""
+        p = LaunchParams(refill, chunk, int(prefer_wide), spec_slack, leaf_slack, stack_lds)
         self.check(self.lib.rtq_launch_params_set(self.h, ctypes.byref(p)))
 
     def autotune(self, items, reps: int = 3) -> TuneResult:

@@ -126,8 +126,11 @@ def test_autotune_cache_roundtrip(rtq_ctx, tmp_path):
         assert res.n_tried > 10 and res.from_cache == 0
         assert res.best_ms <= res.default_ms
         got = rtq_ctx.launch_params()
-        assert (got.refill, got.chunk, got.prefer_wide, got.spec_slack, got.leaf_slack) == (
-            res.best.refill, res.best.chunk, res.best.prefer_wide, res.best.spec_slack, res.best.leaf_slack)
+        assert (got.refill, got.chunk, got.prefer_wide, got.spec_slack, got.leaf_slack, got.stack_lds) == (
+            res.best.refill, res.best.chunk, res.best.prefer_wide, res.best.spec_slack, res.best.leaf_slack,
+            res.best.stack_lds)
+        # the chosen LDS stack depth is named, from the traced layout's choices
+        assert res.best.stack_lds in ((8, 16, 20) if res.best.prefer_wide else (8, 12, 14))
         check_same(wl.hits(), wl.solo(), wl.flags)         # tuned settings give the same hits
         again = rtq_ctx.autotune(wl.items(), reps=1)
         assert again.from_cache == 1 and again.n_tried == 0
@@ -161,7 +164,71 @@ def test_launch_params_validation(rtq_ctx):
         rtq_ctx.set_launch_params(8, 128, True, 64)
     with pytest.raises(RtqError):
         rtq_ctx.set_launch_params(8, 128, True, 4, 64)
+    for depth in (4, 10, 24, 32):
+        with pytest.raises(RtqError):
+            rtq_ctx.set_launch_params(8, 128, True, 4, 8, depth)
+    for depth in (0, 8, 12, 14, 16, 20):
+        rtq_ctx.set_launch_params(8, 128, True, 4, 8, depth)
+        assert rtq_ctx.launch_params().stack_lds == depth
     rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack, p.leaf_slack)
+
+
+def needle_scene(n, seed):
+    """Long thin triangles in the unit cube: large boxes around tiny areas, so
+    rays cross many boxes without a hit and traversal stacks grow deep."""
+    rng = np.random.default_rng(seed)
+    p0 = rng.uniform(0, 1, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    p1 = p0 + 0.6 * d
+    p2 = p1 + rng.uniform(-2e-3, 2e-3, (n, 3))
+    return scenes.triangles_from_vertices(p0.astype(np.float32), p1.astype(np.float32), p2.astype(np.float32))
+
+
+@pytest.mark.parametrize("node_width", [2, 8])
+def test_stack_depths_same_hits(rtq_ctx, torch_mod, node_width):
+    """Every LDS stack depth a batched launch can take (the rest of the stack
+    in HBM) gives the records of the single-launch trace and of brute force,
+    eagerly and replayed from a graph."""
+    tris = needle_scene(20_000, 3)
+    sc = rtq_ctx.upload(tris, scenes.identity_models())
+    p = rtq_ctx.launch_params()
+    try:
+        sc.build(16, 4, 0, node_width=node_width)
+        rays = scenes.random_rays(16_384, 5, (-0.2, -0.2, -0.2), (1.2, 1.2, 1.2))
+        d_rays = devmem.to_device(rays)
+        want_c = sc.trace_host(rays, 0)
+        want_a = sc.trace_host(rays, RTQ_TRACE_ANY)
+        assert (want_c["id"] >= 0).mean() > 0.05
+        v9 = oracle.world_vertices(tris, scenes.identity_models())
+        assert hits_equal(want_c[:2000], oracle.trace_brute(v9, rays[:2000]))
+        hc, ha = devmem.empty(len(rays), HIT_DTYPE), devmem.empty(len(rays), HIT_DTYPE)
+        items = rtq_ctx.items([(sc, d_rays.data_ptr(), len(rays), hc.data_ptr(), 0),
+                               (sc, d_rays.data_ptr(), len(rays), ha.data_ptr(), RTQ_TRACE_ANY)])
+        wide = node_width == 8
+        for depth in ((8, 16, 20) if wide else (8, 12, 14)):
+            rtq_ctx.set_launch_params(p.refill, p.chunk, wide, p.spec_slack, p.leaf_slack, depth)
+            hc.fill_(255)
+            ha.fill_(255)
+            torch_mod.cuda.synchronize()
+            rtq_ctx.trace_batch(items)
+            rtq_ctx.synchronize()
+            check_same([devmem.to_host(hc, HIT_DTYPE), devmem.to_host(ha, HIT_DTYPE)], [want_c, want_a],
+                       [0, RTQ_TRACE_ANY])
+            if depth == 8:
+                g = rtq_ctx.batch(items)
+                try:
+                    for _ in range(2):
+                        hc.fill_(255)
+                        torch_mod.cuda.synchronize()
+                        g.launch()
+                        rtq_ctx.synchronize()
+                        assert hits_equal(devmem.to_host(hc, HIT_DTYPE), want_c)
+                finally:
+                    g.destroy()
+    finally:
+        rtq_ctx.set_launch_params(p.refill, p.chunk, bool(p.prefer_wide), p.spec_slack, p.leaf_slack)
+        sc.release()
 
 
 @pytest.mark.parametrize("spec,leaf", [(0, 0), (1, 63), (16, 1), (63, 8), (4, 8)])

@@ -207,7 +207,11 @@ rtq_status rtq_morton_codes(rtq_ctx* ctx, uint32_t scene_id, uint32_t flags, uin
 
 /* Stable LSD radix sort of (key, value) pairs by key on device memory, in
  * place; with iota values it equals std::sort on (code, index) pairs
- * (bvh.cpp:223-231).  key_bits <= 32 selects how many low key bits are sorted. */
+ * (bvh.cpp:223-231).  key_bits <= 32 selects how many low key bits are sorted;
+ * n < 2^30.  One launch counts the digits of every pass, then one launch per
+ * 8-bit digit finds its offsets by decoupled look-back and scatters
+ * (the single-pass digit binning chainedScanDigitBinning.glsl:31-32 leaves
+ * empty). */
 rtq_status rtq_sort_pairs(rtq_ctx* ctx, uint32_t* keys_dev, uint32_t* vals_dev, uint64_t n,
                           uint32_t key_bits, void* stream);
 

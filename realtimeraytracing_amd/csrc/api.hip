@@ -614,13 +614,19 @@ rtq_status rtq_sort_pairs(rtq_ctx* ctx, uint32_t* keys_dev, uint32_t* vals_dev, 
     const hipStream_t st = pick_stream(ctx, stream);
     {
         Temps tmp(ctx);
-        uint32_t *tk = nullptr, *tv = nullptr;
+        uint32_t *tk = nullptr, *tv = nullptr, *err = nullptr;
         RTQ_HIP(ctx, tmp.get(&tk, n));
         RTQ_HIP(ctx, tmp.get(&tv, n));
-        s = radix_sort_pairs(ctx, keys_dev, vals_dev, tk, tv, n, key_bits, st);
+        RTQ_HIP(ctx, tmp.get(&err, 1));
+        RTQ_HIP(ctx, hipMemsetAsync(err, 0, sizeof(uint32_t), st));
+        s = radix_sort_pairs(ctx, keys_dev, vals_dev, tk, tv, n, key_bits, err, st);
+        uint32_t h_err = 0;
+        hipError_t e = hipMemcpyAsync(&h_err, err, sizeof h_err, hipMemcpyDeviceToHost, st);
         // the temporaries go back to the arena once the sort is done
-        const hipError_t e = hipStreamSynchronize(st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = e2;
         if (s == RTQ_OK && e != hipSuccess) s = hip_fail(ctx, e, "hipStreamSynchronize");
+        if (s == RTQ_OK && h_err) s = fail(ctx, RTQ_EINTERNAL, "sort_pairs: a digit pass's look-back gave up");
     }
     return s;
 }

@@ -115,32 +115,64 @@ __global__ __launch_bounds__(kSmallThreads) void k_refit_merge_small(const uint2
 // appended, so each node is read once and written with its refs into the set
 // already rewritten to kTopBit | slot (leaf refs and refs leaving the set
 // unchanged).
-__global__ void k_top_nodes(const float4* __restrict__ nodes, float4* __restrict__ top,
-                            uint32_t* __restrict__ n_top) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t q[kTopNodes > 0 ? kTopNodes : 1];
-    uint32_t n = 0;
-    q[n++] = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const float4* src = nodes + 4ull * q[i];
-        const float4 a = src[0], b = src[1], c = src[2];
-        float4 w = src[3];
-        uint32_t r[2] = {__float_as_uint(w.x), __float_as_uint(w.y)};
-        for (int k = 0; k < 2; ++k)
-            if (!(r[k] & kLeafBit) && n < kTopNodes) {
-                q[n] = r[k];
-                r[k] = kTopBit | n;
-                ++n;
-            }
-        w.x = __uint_as_float(r[0]);
-        w.y = __uint_as_float(r[1]);
-        float4* dst = top + 4ull * i;
-        dst[0] = a;
-        dst[1] = b;
-        dst[2] = c;
-        dst[3] = w;
+// One wave, level by level: lane l holds BFS slot l.  A level's lanes load
+// their nodes together, then number their internal children in (slot, left
+// before right) order with a wave prefix sum, which is the order a FIFO
+// breadth-first walk appends them; children past kTopNodes keep their refs.
+static_assert(kTopNodes <= 64, "one lane per top-of-tree slot");
+__global__ __launch_bounds__(64) void k_top_nodes(const float4* __restrict__ nodes, float4* __restrict__ top,
+                                                  uint32_t* __restrict__ n_top) {
+    __shared__ uint32_t s_q[64];
+    const uint32_t lane = threadIdx.x;
+    uint32_t node = 0;                 // node index of slot `lane` once its level is reached
+    uint32_t begin = 0, end = 1;       // the current level's slots
+    while (begin < end) {
+        const bool mine = lane >= begin && lane < end;
+        float4 a{}, b{}, c{}, w{};
+        uint32_t r0 = kLeafBit, r1 = kLeafBit;
+        if (mine) {
+            const float4* src = nodes + 4ull * node;
+            a = src[0];
+            b = src[1];
+            c = src[2];
+            w = src[3];
+            r0 = __float_as_uint(w.x);
+            r1 = __float_as_uint(w.y);
+        }
+        const uint32_t k0 = (r0 & kLeafBit) ? 0u : 1u, k1 = (r1 & kLeafBit) ? 0u : 1u;
+        uint32_t incl = k0 + k1;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= (uint32_t)d) incl += o;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t s0 = end + incl - (k0 + k1), s1 = s0 + k0;
+        if (k0 && s0 < kTopNodes) {
+            s_q[s0] = r0;
+            r0 = kTopBit | s0;
+        }
+        if (k1 && s1 < kTopNodes) {
+            s_q[s1] = r1;
+            r1 = kTopBit | s1;
+        }
+        if (mine) {
+            w.x = __uint_as_float(r0);
+            w.y = __uint_as_float(r1);
+            float4* dst = top + 4ull * lane;
+            dst[0] = a;
+            dst[1] = b;
+            dst[2] = c;
+            dst[3] = w;
+        }
+        __syncthreads();
+        const uint32_t next = min(end + total, kTopNodes);
+        if (lane >= end && lane < next) node = s_q[lane];
+        __syncthreads();
+        begin = end;
+        end = next;
     }
-    *n_top = n;
+    if (lane == 0) *n_top = end;
 }
 
 // ------------------------------------------------------------- scene bounds
@@ -314,7 +346,8 @@ __global__ __launch_bounds__(kThreads) void k_leaf_init(const rtq_triangle* __re
                                                         Box6* __restrict__ b_in) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    const uint32_t ti = sorted_idx[i];
+    // in range even after a failed sort (which fails the build)
+    const uint32_t ti = min(sorted_idx[i], n - 1);
     const rtq_triangle t = tris[ti];
     Box6 b;
     tri_box(t, models[t.model_id], b.lo, b.hi);
@@ -509,8 +542,9 @@ __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* s_
 
 // Merge of clusters cid (box b) and cj (box bj) into cluster id, as
 // k_ploc_merge writes it; b becomes the merged box.
-__device__ __forceinline__ void ploc_merge_write(const PlocClusters& cl, uint32_t id, uint32_t cid, uint32_t cj,
-                                                 Box6& b, const Box6& bj) {
+// Merged cluster id of (cid, cj) with triangle count `count` (the sum of theirs).
+__device__ __forceinline__ void ploc_merge_store(const PlocClusters& cl, uint32_t id, uint32_t cid, uint32_t cj,
+                                                 uint32_t count, Box6& b, const Box6& bj) {
     Box6 m;
     rtq_aabb a{};
 #pragma unroll
@@ -525,9 +559,17 @@ __device__ __forceinline__ void ploc_merge_write(const PlocClusters& cl, uint32_
     cl.right[id] = cj;
     cl.parent[cid] = id;
     cl.parent[cj] = id;
-    cl.count[id] = cl.count[cid] + cl.count[cj];
+    cl.count[id] = count;
     cl.tri[id] = 0;
     b = m;
+}
+
+__device__ __forceinline__ void ploc_merge_write(const PlocClusters& cl, uint32_t id, uint32_t cid, uint32_t cj,
+                                                 Box6& b, const Box6& bj) {
+    // the counts are read before any store: the arrays may alias as far as
+    // the compiler knows, and a load after a store waits for the store
+    const uint32_t count = cl.count[cid] + cl.count[cj];
+    ploc_merge_store(cl, id, cid, cj, count, b, bj);
 }
 
 struct PlocPassArgs {
@@ -553,7 +595,6 @@ __global__ __launch_bounds__(kPassThreads) void k_ploc_pass(PlocPassArgs a) {
     const int r = (int)a.radius;
     const uint32_t T = (uint32_t)(kPassThreads - 2 * r);
     if (tid == 0) {
-        const uint32_t tile = atomicAdd(a.tile_ctr, 1u);
         const uint32_t iter = a.st_in[0], n_total = a.st_in[1], cur = a.st_in[2] & 1u;
         uint32_t skip = 0;
         if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || iter <= kTailMax) {
@@ -562,13 +603,18 @@ __global__ __launch_bounds__(kPassThreads) void k_ploc_pass(PlocPassArgs a) {
             atomicOr(a.err, kPlocErrBound);
             skip = 1;
         }
-        if (skip && tile == 0) {  // left to the tail kernel (or stopped): carry the state over
+        if (skip && blockIdx.x == 0) {  // left to the tail kernel (or stopped): carry the state over
             a.st_out[0] = iter;
             a.st_out[1] = n_total;
             a.st_out[2] = cur;
             *a.log = make_uint4(n_total, 0u, iter, 1u);
         }
-        if (!skip && (uint64_t)tile * T >= iter) skip = 1;
+        // the grid covers the batch's first pass: only as many workgroups as
+        // this pass has tiles take one (the others leave without touching the
+        // counter, whose atomics serialise), so the tiles are 0 .. need - 1
+        const uint32_t need = (iter + T - 1) / T;
+        if (!skip && blockIdx.x >= need) skip = 1;
+        const uint32_t tile = skip ? 0u : atomicAdd(a.tile_ctr, 1u);
         s_hdr[0] = tile;
         s_hdr[1] = iter;
         s_hdr[2] = n_total;
@@ -597,10 +643,13 @@ __global__ __launch_bounds__(kPassThreads) void k_ploc_pass(PlocPassArgs a) {
             const Box6 bi = s_box[tid + r];
             const int start = max(i - r, 0), end = min(i + r + 1, (int)iter);
             float best = INFINITY;
+            // branch-free body, unrolled: the LDS reads of several j overlap
+#pragma unroll 4
             for (int j = start; j < end; ++j) {
-                if (j == i) continue;
                 const float d = merged_sa(bi, s_box[j - lo]);
-                if (d < best) { best = d; arg = (uint32_t)j; }
+                const bool take = j != i && d < best;
+                best = take ? d : best;
+                arg = take ? (uint32_t)j : arg;
             }
             if (arg == kNone) atomicOr(a.err, kPlocErrNoNeighbour);
         }
@@ -715,6 +764,7 @@ __global__ __launch_bounds__(kTailThreads) void k_ploc_tail(uint32_t* __restrict
                                                             PlocClusters cl) {
     __shared__ Box6 s_box[kTailMax];
     __shared__ uint32_t s_c[kTailMax];
+    __shared__ uint32_t s_n[kTailMax];   // triangle counts: no global loads between the passes' barriers
     __shared__ uint16_t s_nn[kTailMax];
     __shared__ uint32_t s_w[kTailThreads / 64];
     __shared__ uint32_t s_hdr[4];
@@ -735,7 +785,9 @@ __global__ __launch_bounds__(kTailThreads) void k_ploc_tail(uint32_t* __restrict
     const uint32_t cur = s_hdr[2];
     for (uint32_t q = tid; q < m; q += kTailThreads) {
         s_box[q] = lists.b[cur][q];
-        s_c[q] = lists.c[cur][q];
+        const uint32_t c = lists.c[cur][q];
+        s_c[q] = c;
+        s_n[q] = cl.count[c];
     }
     __syncthreads();
     uint32_t passes = 0;
@@ -745,10 +797,12 @@ __global__ __launch_bounds__(kTailThreads) void k_ploc_tail(uint32_t* __restrict
             const int start = max(i - r, 0), end = min(i + r + 1, (int)m);
             float best = INFINITY;
             uint16_t arg = kNone16;
+#pragma unroll 4
             for (int j = start; j < end; ++j) {
-                if (j == i) continue;
                 const float d = merged_sa(bi, s_box[j]);
-                if (d < best) { best = d; arg = (uint16_t)j; }
+                const bool take = j != i && d < best;
+                best = take ? d : best;
+                arg = take ? (uint16_t)j : arg;
             }
             if (arg == kNone16) atomicOr(err, kPlocErrNoNeighbour);
             s_nn[i] = arg;
@@ -769,21 +823,23 @@ __global__ __launch_bounds__(kTailThreads) void k_ploc_tail(uint32_t* __restrict
         uint32_t total = 0;
         uint32_t run = block_exclusive_sum<kTailThreads>(sum, s_w, total);
         Box6 ob[4];
-        uint32_t oc[4], opos[4];
+        uint32_t oc[4], on[4], opos[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t i = 4u * tid + k;
             opos[k] = kNone;
             if (i < m && (f[k] & 0xFFFFu)) {
-                uint32_t cid = s_c[i];
+                uint32_t cid = s_c[i], cnt = s_n[i];
                 Box6 b = s_box[i];
                 if (f[k] >> 16) {
                     const uint32_t j = s_nn[i], id = n_total + (run >> 16);
-                    ploc_merge_write(cl, id, cid, s_c[j], b, s_box[j]);
+                    cnt += s_n[j];
+                    ploc_merge_store(cl, id, cid, s_c[j], cnt, b, s_box[j]);
                     cid = id;
                 }
                 ob[k] = b;
                 oc[k] = cid;
+                on[k] = cnt;
                 opos[k] = run & 0xFFFFu;
             }
             run += f[k];
@@ -794,6 +850,7 @@ __global__ __launch_bounds__(kTailThreads) void k_ploc_tail(uint32_t* __restrict
             if (opos[k] != kNone) {
                 s_box[opos[k]] = ob[k];
                 s_c[opos[k]] = oc[k];
+                s_n[opos[k]] = on[k];
             }
         const uint32_t merges = total >> 16, stays = total & 0xFFFFu;
         if (tid == 0) log[passes] = make_uint4(n_total, merges, m, 0u);
@@ -1023,10 +1080,12 @@ rtq_status compute_bounds(rtq_ctx* ctx, Scene* s, uint32_t pad_origin, rtq_aabb*
 // PLOC with fused passes: batches of kPlocBatch k_ploc_pass launches (each a
 // no-op once at most kTailMax clusters are left) and one k_ploc_tail launch
 // (a no-op until then), then one read of the state and the pass logs.  The
-// active list starts in (c0, b0); c1, b1 are the other buffers.
+// active list starts in (c0, b0); c1, b1 are the other buffers.  A batch's
+// passes all take the grid of its first pass; 16 passes per batch bring a
+// 2M-triangle terrain (61 passes) to the tail in two host round trips.
 rtq_status ploc_fused(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint32_t* c0, uint32_t* c1,
                       Box6* b0, Box6* b1, std::vector<std::pair<uint32_t, uint32_t>>& passes, hipStream_t st) {
-    constexpr uint32_t kPlocBatch = 8;
+    constexpr uint32_t kPlocBatch = 16;
     const uint32_t n = s->n_tris;
     const uint32_t T = kPassThreads - 2 * radius;
     const uint32_t max_tiles = (n + T - 1) / T;
@@ -1035,9 +1094,12 @@ rtq_status ploc_fused(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint3
     uint32_t *ctl = nullptr, *state = nullptr, *err = nullptr, *tail_count = nullptr;
     uint4 *log = nullptr, *tail_log = nullptr;
     RTQ_HIP(ctx, tmp.get(&ctl, ctl_words));
-    RTQ_HIP(ctx, tmp.get(&state, 8));
-    RTQ_HIP(ctx, tmp.get(&err, 2));
-    RTQ_HIP(ctx, tmp.get(&log, kPlocBatch));
+    // state (8 words), error and tail pass count (8), pass log: one block,
+    // read back with one copy per batch
+    constexpr uint32_t kHeadWords = 16 + 4 * kPlocBatch;
+    RTQ_HIP(ctx, tmp.get(&state, kHeadWords));
+    err = state + 8;
+    log = reinterpret_cast<uint4*>(state + 16);
     RTQ_HIP(ctx, tmp.get(&tail_log, kTailMax));
     tail_count = err + 1;
     const uint32_t h_state0[8] = {n, n, 0u, 0u, 0u, 0u, 0u, 0u};
@@ -1064,12 +1126,13 @@ rtq_status ploc_fused(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint3
         hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(kTailThreads), 0, st, state, radius, err, tail_log, tail_count,
                            lists, cl);
         RTQ_HIP(ctx, hipGetLastError());
-        uint4 h_log[kPlocBatch];
-        uint32_t h_state[4], h_err[2];
-        RTQ_HIP(ctx, hipMemcpyAsync(h_log, log, sizeof h_log, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipMemcpyAsync(h_state, state, sizeof h_state, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipMemcpyAsync(h_err, err, sizeof h_err, hipMemcpyDeviceToHost, st));
+        uint32_t h_head[kHeadWords];
+        RTQ_HIP(ctx, hipMemcpyAsync(h_head, state, sizeof h_head, hipMemcpyDeviceToHost, st));
         RTQ_HIP(ctx, hipStreamSynchronize(st));
+        const uint32_t* h_state = h_head;
+        const uint32_t* h_err = h_head + 8;
+        uint4 h_log[kPlocBatch];
+        memcpy(h_log, h_head + 16, sizeof h_log);
         if (h_err[0] & kPlocErrNoMerge) return fail(ctx, RTQ_EINTERNAL, "PLOC pass made no merge");
         if (h_err[0])
             return fail(ctx, RTQ_EINTERNAL, "fused PLOC pass failed (error bits 0x%x)", h_err[0]);
@@ -1282,10 +1345,11 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     // triangle: sort keys and values and the PLOC lists 64 B, placement 32 B;
     // with 8-wide nodes also the collapse cost table 36 B and the collapse
     // buffers 52 B per binary traversal node (one per triangle with leaves of
-    // one triangle); each placement workgroup's deepest leaf n / 32 B; 1 MiB
-    // for scans and 256-B rounding
+    // one triangle); the fused PLOC passes' tile status words (16 passes x
+    // 8 B per tile of 480) and each placement workgroup's deepest leaf, under
+    // n / 2 B; 1 MiB for scans and 256-B rounding
     const size_t per_tri = prm.node_width == 8 ? 64 + 32 + 36 + 52 : 64 + 32;
-    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + n / 32 + ((size_t)1 << 20));
+    ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + n / 2 + ((size_t)1 << 20));
     rtq_status rs = arena_reserve(ctx);
     if (rs != RTQ_OK) return rs;
 
@@ -1320,6 +1384,9 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     RTQ_HIP(ctx, tmp.get(&c_out, n));
     RTQ_HIP(ctx, tmp.get(&b_in, n));
     RTQ_HIP(ctx, tmp.get(&b_out, n));
+    uint32_t* sort_err = nullptr;
+    RTQ_HIP(ctx, tmp.get(&sort_err, 1));
+    RTQ_HIP(ctx, hipMemsetAsync(sort_err, 0, sizeof(uint32_t), st));
 
     // 1. bounds, cube, Morton codes, sort
     rs = compute_bounds(ctx, s, prm.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
@@ -1327,7 +1394,7 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, n,
                        d_bounds, s->sorted_codes, s->sorted_idx);
     RTQ_HIP(ctx, hipGetLastError());
-    rs = radix_sort_pairs(ctx, s->sorted_codes, s->sorted_idx, tk, tv, n, 30, st);
+    rs = radix_sort_pairs(ctx, s->sorted_codes, s->sorted_idx, tk, tv, n, 30, sort_err, st);
     if (rs != RTQ_OK) return rs;
 
     // 2. PLOC
@@ -1362,7 +1429,10 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     rs = write_layouts(ctx, s, d_bounds, st);
     if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipEventRecord(ev1, st));
+    uint32_t h_sort_err = 0;
+    RTQ_HIP(ctx, hipMemcpyAsync(&h_sort_err, sort_err, sizeof h_sort_err, hipMemcpyDeviceToHost, st));
     RTQ_HIP(ctx, hipStreamSynchronize(st));
+    if (h_sort_err) return fail(ctx, RTQ_EINTERNAL, "build: a radix sort pass's look-back gave up");
     float ms = 0.0f;
     RTQ_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
     s->build_ms = ms;
@@ -1377,15 +1447,14 @@ rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st) {
     if (kTopNodes == 0) return RTQ_OK;
     // the set depends on the topology only, so a refit or an order update
     // rewrites the same buffer with the same node count
-    if (!s->top) RTQ_HIP(ctx, dev_alloc(&s->top, 4ull * kTopNodes));
-    uint32_t* d_n = nullptr;
-    RTQ_HIP(ctx, dev_alloc(&d_n, 1));
-    hipLaunchKernelGGL(k_top_nodes, dim3(1), dim3(1), 0, st, s->nodes, s->top, d_n);
+    // one float4 past the nodes holds the node count
+    if (!s->top) RTQ_HIP(ctx, dev_alloc(&s->top, 4ull * kTopNodes + 1));
+    uint32_t* d_n = reinterpret_cast<uint32_t*>(s->top + 4ull * kTopNodes);
+    hipLaunchKernelGGL(k_top_nodes, dim3(1), dim3(64), 0, st, s->nodes, s->top, d_n);
     hipError_t e = hipGetLastError();
     uint32_t n = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&n, d_n, sizeof n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(d_n);
     if (e != hipSuccess) return hip_fail(ctx, e, "write_top");
     s->n_top = n;
     return RTQ_OK;

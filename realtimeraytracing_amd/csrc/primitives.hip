@@ -2,11 +2,12 @@
 //
 // Scan: reduce-then-scan over 2048-element tiles (256 threads x 8 items,
 // blocked), tile sums scanned recursively, then a block scan per tile.
-// Radix sort: 8-bit digits, one histogram + one scatter launch per digit.  The
-// scatter ranks keys stably inside a tile with wave-wide digit match masks
-// (8 ballots per key) and per-wave digit counts in LDS, so equal keys keep
-// their input order: with iota values this is std::sort on (code, index)
-// pairs (bvh.cpp:223-231).  The reference's own GPU sort never got past a
+// Radix sort: 8-bit digits; one launch counts every digit position, then one
+// Onesweep launch per digit (decoupled look-back over tiles for the digit
+// offsets, then the scatter).  The scatter ranks keys stably inside a tile
+// with wave-wide digit match masks (8 ballots per key) and per-wave digit
+// counts in LDS, so equal keys keep their input order: with iota values this
+// is std::sort on (code, index) pairs (bvh.cpp:223-231).  The reference's own GPU sort never got past a
 // set-bit histogram and a per-group scan (histogramOfGlobalDigitCounts.glsl,
 // prefixSumOfGlobalDigitCounts.glsl, chainedScanDigitBinning.glsl:31-32 empty).
 #include "rtq_internal.hpp"
@@ -168,70 +169,165 @@ rtq_status scan_impl_ws(rtq_ctx* ctx, const T* in, T* out, uint64_t n, T* total,
 // ---------------------------------------------------------------- radix sort
 
 constexpr int kSortThreads = 256;
-constexpr int kSortRounds = 8;
-constexpr int kSortTile = kSortThreads * kSortRounds;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortThreads * kSortRounds;   // keys per tile
+constexpr int kSortHistBlocks = 512;
+// Look-back status word of (tile, digit): flags in the top two bits, a count
+// (below 2^30) in the others.  Aggregate = the tile's own count of the digit,
+// prefix = its count plus every earlier tile's.
+constexpr uint32_t kDigitAgg = 1u << 30;
+constexpr uint32_t kDigitPre = 2u << 30;
+constexpr uint32_t kDigitFlags = 3u << 30;
+constexpr uint32_t kSortSpinLimit = 1u << 24;   // ~0.5 s of s_sleep(1) polls
 
-__global__ __launch_bounds__(kSortThreads) void k_digit_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                             uint32_t shift, uint32_t n_tiles,
-                                                             uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_h[256];
-    s_h[threadIdx.x] = 0;
+// Counts of every digit at every digit position in one read of the keys:
+// hist[p * 256 + d] = keys whose digit p (bits 8p .. 8p+7) is d, p < n_digits.
+// Each workgroup counts its tiles in LDS and adds its non-zero bins.
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                            uint32_t n_digits, uint32_t n_tiles,
+                                                            uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[4][256];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) s_h[p][threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; ++r) {
-        uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 0xFFu], 1u);
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint32_t base = t * kSortTile;
+        for (int r = 0; r < kSortRounds; ++r) {
+            const uint32_t i = base + r * kSortThreads + threadIdx.x;
+            if (i < n) {
+                const uint32_t k = keys[i];
+                for (uint32_t p = 0; p < n_digits; ++p) atomicAdd(&s_h[p][(k >> (8 * p)) & 0xFFu], 1u);
+            }
+        }
     }
     __syncthreads();
-    hist[threadIdx.x * n_tiles + blockIdx.x] = s_h[threadIdx.x];
+    for (uint32_t p = 0; p < n_digits; ++p)
+        if (s_h[p][threadIdx.x]) atomicAdd(&hist[p * 256 + threadIdx.x], s_h[p][threadIdx.x]);
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_digit_scatter(
+// Exclusive scan of each digit position's 256 counts: base[p * 256 + d] =
+// keys whose digit p is below d.  One workgroup.
+__global__ __launch_bounds__(kSortThreads) void k_sort_digit_base(const uint32_t* __restrict__ hist,
+                                                                  uint32_t n_digits, uint32_t* __restrict__ base) {
+    __shared__ uint32_t s_w[kSortThreads / 64];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    for (uint32_t p = 0; p < n_digits; ++p) {
+        const uint32_t c = hist[p * 256 + threadIdx.x];
+        uint32_t x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == 63) s_w[wid] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t w = 0; w < wid; ++w) before += s_w[w];
+        base[p * 256 + threadIdx.x] = before + x - c;
+        __syncthreads();
+    }
+}
+
+// One digit pass in one launch (Adinets & Merrill, "Onesweep: A Faster Least
+// Significant Digit Radix Sort for GPUs", 2022): a workgroup takes the next
+// tile (atomic counter, so every tile it waits on is already running),
+// counts the tile's digits in LDS, publishes them and sums the counts of the
+// earlier tiles by decoupled look-back per digit (Merrill & Garland 2016),
+// then scatters each key to digit_base[d] + earlier tiles' count + its rank
+// in the tile.  Ranks are stable: rounds in order, waves in order, lanes in
+// order (wave-wide digit match masks, 8 ballots per key).  A look-back that
+// waits longer than kSortSpinLimit polls sets *err and goes on.
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n, uint32_t shift,
-    uint32_t n_tiles, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ out_k,
-    uint32_t* __restrict__ out_v) {
+    const uint32_t* __restrict__ digit_base, uint32_t* __restrict__ tile_ctr, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ err, uint32_t* __restrict__ out_k, uint32_t* __restrict__ out_v) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_cnt[256];
     __shared__ uint32_t s_off[256];
     __shared__ uint32_t s_wcnt[kSortThreads / 64][256];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    s_off[threadIdx.x] = offsets[threadIdx.x * n_tiles + blockIdx.x];
-    const uint32_t base = blockIdx.x * kSortTile;
+    const uint32_t tid = threadIdx.x, wid = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    s_cnt[tid] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * kSortTile;
+    uint32_t key[kSortRounds], val[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + tid;
+        key[r] = 0;
+        val[r] = 0;
+        if (i < n) {
+            key[r] = keys[i];
+            val[r] = vals[i];
+            atomicAdd(&s_cnt[(key[r] >> shift) & 0xFFu], 1u);
+        }
+    }
+    __syncthreads();
+    // thread d: digit d's count through this tile
+    const uint32_t c = s_cnt[tid];
+    uint32_t* const mine = status + (size_t)tile * 256 + tid;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        __hip_atomic_store(mine, kDigitPre | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(mine, kDigitAgg | c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t k = tile - 1, spins = 0;
+        for (;;) {
+            const uint32_t w = __hip_atomic_load(status + (size_t)k * 256 + tid, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t f = w & kDigitFlags;
+            if (f == 0) {
+                if (++spins > kSortSpinLimit) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += w & ~kDigitFlags;
+            if (f == kDigitPre) break;
+            if (k == 0) {   // tile 0 publishes a prefix: a broken invariant
+                atomicOr(err, 1u);
+                break;
+            }
+            --k;
+        }
+        __hip_atomic_store(mine, kDigitPre | (excl + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_off[tid] = digit_base[tid] + excl;
+    __syncthreads();
+#pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
 #pragma unroll
-        for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][threadIdx.x] = 0;
+        for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][tid] = 0;
         __syncthreads();
-        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        const uint32_t i = base + r * kSortThreads + tid;
         const bool valid = i < n;
-        uint32_t key = 0, val = 0, d = 0;
-        if (valid) {
-            key = keys[i];
-            val = vals[i];
-            d = (key >> shift) & 0xFFu;
-        }
+        const uint32_t d = (key[r] >> shift) & 0xFFu;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(valid && bit);
+            const uint64_t bb = __ballot(valid && bit);
             m &= bit ? bb : ~bb;
         }
-        const uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         if (valid && rank == 0) s_wcnt[wid][d] = (uint32_t)__popcll(m);
         __syncthreads();
         if (valid) {
             uint32_t pos = s_off[d] + rank;
-            for (int w = 0; w < wid; ++w) pos += s_wcnt[w][d];
-            out_k[pos] = key;
-            out_v[pos] = val;
+            for (uint32_t w = 0; w < wid; ++w) pos += s_wcnt[w][d];
+            out_k[pos] = key[r];
+            out_v[pos] = val[r];
         }
         __syncthreads();
         uint32_t add = 0;
 #pragma unroll
-        for (int w = 0; w < kSortThreads / 64; ++w) add += s_wcnt[w][threadIdx.x];
-        s_off[threadIdx.x] += add;
+        for (int w = 0; w < kSortThreads / 64; ++w) add += s_wcnt[w][tid];
+        s_off[tid] += add;
         __syncthreads();
     }
-    (void)lane;
 }
 
 }  // namespace
@@ -251,45 +347,41 @@ rtq_status scan_u32x2_ws(rtq_ctx* ctx, const uint2* in, uint2* out, uint64_t n, 
 }
 
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
-                            uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st) {
+                            uint32_t* tmp_v, uint64_t n, uint32_t key_bits, uint32_t* err, hipStream_t st) {
     if (n == 0 || key_bits == 0) return RTQ_OK;
-    if (n >= (1ull << 31)) return fail(ctx, RTQ_ECAPACITY, "sort: n must be < 2^31");
+    if (n >= (1ull << 30)) return fail(ctx, RTQ_ECAPACITY, "sort: n must be < 2^30");
     const uint32_t nn = (uint32_t)n;
     const uint32_t n_tiles = (nn + kSortTile - 1) / kSortTile;
-    // digit counts, their offsets and the offsets scan's workspace in one
-    // block, so the passes queue without synchronising
-    const uint64_t n_hist = 256ull * n_tiles;
-    const uint64_t ws = scan_workspace<U1>(n_hist);
+    const uint32_t passes = (key_bits + 7) / 8;
+    // digit counts and bases of every pass, the passes' tile counters and one
+    // pass's look-back status words (cleared before each pass): the passes
+    // queue without synchronising
     Temps tmp(ctx);
-    uint32_t* hist = nullptr;
-    RTQ_HIP(ctx, tmp.get(&hist, 2ull * n_hist + ws));
-    uint32_t* offs = hist + n_hist;
-    U1* scan_ws = reinterpret_cast<U1*>(offs + n_hist);
+    uint32_t *hist = nullptr, *dbase = nullptr, *ctr = nullptr, *status = nullptr;
+    RTQ_HIP(ctx, tmp.get(&status, (size_t)n_tiles * 256));
+    RTQ_HIP(ctx, tmp.get(&hist, 3 * 1024 + 64));
+    dbase = hist + 1024;
+    ctr = hist + 2048;
+    RTQ_HIP(ctx, hipMemsetAsync(hist, 0, sizeof(uint32_t) * (3 * 1024 + 64), st));
+    hipLaunchKernelGGL(k_sort_hist, dim3(std::min<uint32_t>(n_tiles, kSortHistBlocks)), dim3(kSortThreads), 0, st,
+                       keys, nn, passes, n_tiles, hist);
+    RTQ_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_sort_digit_base, dim3(1), dim3(kSortThreads), 0, st, hist, passes, dbase);
+    RTQ_HIP(ctx, hipGetLastError());
     uint32_t *src_k = keys, *src_v = vals, *dst_k = tmp_k, *dst_v = tmp_v;
-    rtq_status s = RTQ_OK;
-    int passes = 0;
-    for (uint32_t shift = 0; shift < key_bits && s == RTQ_OK; shift += 8, ++passes) {
-        hipLaunchKernelGGL(k_digit_hist, dim3(n_tiles), dim3(kSortThreads), 0, st, src_k, nn, shift,
-                           n_tiles, hist);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) { s = hip_fail(ctx, e, "k_digit_hist"); break; }
-        s = scan_impl_ws<U1>(ctx, reinterpret_cast<const U1*>(hist), reinterpret_cast<U1*>(offs), n_hist,
-                             nullptr, scan_ws, st);
-        if (s != RTQ_OK) break;
-        hipLaunchKernelGGL(k_digit_scatter, dim3(n_tiles), dim3(kSortThreads), 0, st, src_k, src_v, nn,
-                           shift, n_tiles, offs, dst_k, dst_v);
-        e = hipGetLastError();
-        if (e != hipSuccess) { s = hip_fail(ctx, e, "k_digit_scatter"); break; }
+    for (uint32_t p = 0; p < passes; ++p) {
+        RTQ_HIP(ctx, hipMemsetAsync(status, 0, sizeof(uint32_t) * n_tiles * 256, st));
+        hipLaunchKernelGGL(k_onesweep, dim3(n_tiles), dim3(kSortThreads), 0, st, src_k, src_v, nn, 8 * p,
+                           dbase + 256 * p, ctr + p, status, err, dst_k, dst_v);
+        RTQ_HIP(ctx, hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
     }
-    if (s == RTQ_OK && (passes & 1)) {
-        hipError_t e = hipMemcpyAsync(keys, src_k, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st);
-        if (e != hipSuccess) s = hip_fail(ctx, e, "sort copy-back");
+    if (passes & 1) {
+        RTQ_HIP(ctx, hipMemcpyAsync(keys, src_k, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        RTQ_HIP(ctx, hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
     }
-    return s;
+    return RTQ_OK;
 }
 
 }  // namespace rtq

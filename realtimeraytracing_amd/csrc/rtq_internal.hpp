@@ -297,9 +297,11 @@ rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st);
 // queued on st (build.hip).
 rtq_status ensure_ref_nodes(rtq_ctx* ctx, Scene* s, hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs; `tmp_k`/`tmp_v` are n-entry buffers.
-// Queued on st and not waited for; digit counts come from the context's arena.
+// Queued on st and not waited for; digit counts and look-back words come from
+// the context's arena.  *err (device, zeroed by the caller) becomes non-zero
+// when a pass's look-back gave up: the order is then wrong and the caller fails.
 rtq_status radix_sort_pairs(rtq_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_k,
-                            uint32_t* tmp_v, uint64_t n, uint32_t key_bits, hipStream_t st);
+                            uint32_t* tmp_v, uint64_t n, uint32_t key_bits, uint32_t* err, hipStream_t st);
 
 }  // namespace rtq
 

@@ -125,7 +125,11 @@ __global__ __launch_bounds__(kDpSmallThreads) void k_dp_small(const uint2* __res
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t n_items, const uint32_t* __restrict__ items,
+// A level's node count and first node index are device words (lvl[0], lvl[1])
+// so several levels queue without a host read; the grid covers an upper bound
+// `ub` of the count.
+__global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t ub, const uint32_t* __restrict__ lvl,
+                                                            const uint32_t* __restrict__ items,
                                                             uint32_t max_leaf, const uint32_t* __restrict__ cleft,
                                                             const uint32_t* __restrict__ cright,
                                                             const uint32_t* __restrict__ ccount,
@@ -135,7 +139,11 @@ __global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t n_items, co
                                                             uint32_t* __restrict__ n_children,
                                                             uint32_t* __restrict__ n_expand) {
     const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= n_items) return;
+    if (k >= ub) return;
+    if (k >= lvl[0]) {
+        n_expand[k] = 0;   // the scan covers the bound
+        return;
+    }
     const uint32_t c = items[k];
     uint32_t ch[kWidth];
     uint32_t n = 0;
@@ -183,14 +191,16 @@ __device__ int exponent_for(double extent) {
 }
 
 __global__ __launch_bounds__(kThreads) void k_wide_write(
-    uint32_t n_items, uint32_t level_base, uint32_t next_base, uint32_t max_leaf, float pad,
+    uint32_t ub, const uint32_t* __restrict__ lvl, uint32_t cap, uint32_t* __restrict__ err, uint32_t max_leaf,
+    float pad,
     const uint32_t* __restrict__ items, const uint32_t* __restrict__ children, const uint32_t* __restrict__ n_children,
     const uint32_t* __restrict__ child_offs, const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ ccount,
     const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ fl, const uint32_t* __restrict__ pos,
     const uint32_t* __restrict__ bidx, float4* __restrict__ nodes, uint32_t* __restrict__ wide_trav,
     uint32_t* __restrict__ next_items) {
     const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= n_items) return;
+    const uint32_t n_items = lvl[0], level_base = lvl[1], next_base = level_base + n_items;
+    if (k >= ub || k >= n_items) return;
     // the binary traversal node of this node's root cluster (learned order)
     wide_trav[level_base + k] = bidx ? bidx[pos[items[k]]] : kNone;
     const uint32_t n = n_children[k];
@@ -232,7 +242,11 @@ __global__ __launch_bounds__(kThreads) void k_wide_write(
         if (expandable(c, max_leaf, cleft, ccount)) {
             const uint32_t slot = child_offs[k] + rank++;
             ref[i] = next_base + slot;
-            next_items[slot] = c;
+            if (next_base + slot < cap) {
+                next_items[slot] = c;
+            } else {
+                atomicOr(err, 1u);   // more wide nodes than binary ones: a broken invariant
+            }
         } else {
             ref[i] = leaf_ref(fl[c], ccount[c]);
         }
@@ -256,6 +270,20 @@ __global__ __launch_bounds__(kThreads) void k_wide_write(
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// The next level's count (the scan total) and first node index; a level that
+// would pass the node capacity ends the tree with an error.
+__global__ void k_level_next(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ total, uint32_t cap,
+                             uint32_t* __restrict__ err, uint32_t* __restrict__ next) {
+    const uint32_t base = lvl[1] + lvl[0];
+    uint32_t count = *total;
+    if ((uint64_t)base + count > cap) {
+        *err = 1u;
+        count = 0;
+    }
+    next[0] = count;
+    next[1] = base;
+}
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
 
@@ -304,29 +332,47 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, const uint32_t
             RTQ_HIP(ctx, hipGetLastError());
         }
     }
+    // levels top-down, kLevelsPerRead at a time between host reads: a level's
+    // count and first index stay on the device (lvl), its launches cover an
+    // upper bound (8 children per node, at most cap nodes)
+    constexpr uint32_t kLevelsPerRead = 4;
+    uint32_t* lvl = nullptr;
+    RTQ_HIP(ctx, tmp.get(&lvl, 2 * (kLevelsPerRead + 1) + 1 + kLevelsPerRead));
+    uint32_t* err = lvl + 2 * (kLevelsPerRead + 1);
+    uint32_t* totals = err + 1;
     uint32_t level_items = 1, level_base = 0, levels = 0;
     while (level_items > 0) {
-        if (level_base + level_items > cap) return fail(ctx, RTQ_EINTERNAL, "wide collapse overflow");
-        hipLaunchKernelGGL(k_wide_collapse, dim3(blocks_for(level_items)), dim3(kThreads), 0, st, level_items, items,
-                           s->params.max_leaf_size, s->cleft, s->cright, s->ccount, s->cbox, nt, choice, children,
-                           nchild, nexp);
-        RTQ_HIP(ctx, hipGetLastError());
-        rtq_status rs = scan_u32(ctx, nexp, offs, level_items, total, st);
-        if (rs != RTQ_OK) return rs;
-        uint32_t h_total = 0;
-        RTQ_HIP(ctx, hipMemcpyAsync(&h_total, total, sizeof h_total, hipMemcpyDeviceToHost, st));
+        const uint32_t h_head[3] = {level_items, level_base, 0u};
+        RTQ_HIP(ctx, hipMemcpyAsync(lvl, h_head, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        RTQ_HIP(ctx, hipMemcpyAsync(err, h_head + 2, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        uint64_t ub = level_items;
+        for (uint32_t g = 0; g < kLevelsPerRead; ++g) {
+            const uint32_t u = (uint32_t)ub;
+            uint32_t* lv = lvl + 2 * g;
+            hipLaunchKernelGGL(k_wide_collapse, dim3(blocks_for(u)), dim3(kThreads), 0, st, u, lv, items,
+                               s->params.max_leaf_size, s->cleft, s->cright, s->ccount, s->cbox, nt, choice, children,
+                               nchild, nexp);
+            RTQ_HIP(ctx, hipGetLastError());
+            rtq_status rs = scan_u32(ctx, nexp, offs, u, totals + g, st);
+            if (rs != RTQ_OK) return rs;
+            hipLaunchKernelGGL(k_wide_write, dim3(blocks_for(u)), dim3(kThreads), 0, st, u, lv, cap, err,
+                               s->params.max_leaf_size, s->box_pad, items, children, nchild, offs, s->cleft,
+                               s->ccount, s->cbox, fl, pos, bidx, s->wide, s->wide_trav, next);
+            RTQ_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, st, lv, totals + g, cap, err, lv + 2);
+            RTQ_HIP(ctx, hipGetLastError());
+            std::swap(items, next);
+            ub = std::min<uint64_t>(8 * ub, cap);
+        }
+        uint32_t h_lvl[2 * (kLevelsPerRead + 1) + 1];
+        RTQ_HIP(ctx, hipMemcpyAsync(h_lvl, lvl, sizeof h_lvl, hipMemcpyDeviceToHost, st));
         RTQ_HIP(ctx, hipStreamSynchronize(st));
-        const uint32_t next_base = level_base + level_items;
-        hipLaunchKernelGGL(k_wide_write, dim3(blocks_for(level_items)), dim3(kThreads), 0, st, level_items,
-                           level_base, next_base, s->params.max_leaf_size, s->box_pad, items, children, nchild, offs,
-                           s->cleft, s->ccount, s->cbox, fl, pos, bidx, s->wide, s->wide_trav, next);
-        RTQ_HIP(ctx, hipGetLastError());
-        std::swap(items, next);
-        level_base = next_base;
-        level_items = h_total;
-        ++levels;
+        if (h_lvl[2 * (kLevelsPerRead + 1)]) return fail(ctx, RTQ_EINTERNAL, "wide collapse overflow");
+        for (uint32_t g = 0; g < kLevelsPerRead; ++g)
+            if (h_lvl[2 * g] > 0) ++levels;
+        level_items = h_lvl[2 * kLevelsPerRead];
+        level_base = h_lvl[2 * kLevelsPerRead + 1];
     }
-    RTQ_HIP(ctx, hipStreamSynchronize(st));
     s->n_wide = level_base;
     s->wide_depth = levels;
     return RTQ_OK;

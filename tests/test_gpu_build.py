@@ -212,7 +212,7 @@ def test_terrain_full_size(rtq_ctx):
 
 
 @pytest.mark.parametrize("n,bits", [(1, 32), (2047, 30), (2048, 30), (2049, 32), (100_000, 30), (1_000_003, 32),
-                                    (300_000, 8), (50_000, 12)])
+                                    (300_000, 8), (50_000, 12), (4_194_305, 30), (65_536, 1)])
 def test_radix_sort_pairs(rtq_ctx, torch_mod, n, bits):
     rng = np.random.default_rng(n + bits)
     hi = 1 << bits

@@ -306,22 +306,38 @@ def same_results(ref, got) -> bool:
                 and np.array_equal(rs["id"] >= 0, gs["id"] >= 0))
 
 
+# lights and diffuse seed of the rays the order model trains on: not the timed rays
+TRAIN_LIGHTS = np.array([[0.1, 2.0, 0.5], [0.9, 2.0, 0.5], [0.5, 2.0, 0.1], [0.5, 2.0, 0.9]], np.float32)
+TRAIN_DIFFUSE_SEED = 101
+
+
 def train_order(ctx, wl, stream):
-    """Order model from this scene's own traversal logs (about 200k shadow and
-    100k diffuse rays sampled), applied to the scene's nodes; untimed setup."""
+    """Order model from this scene's own traversal logs, applied to the scene's
+    nodes; untimed setup.  The logs come from held-out rays of the same primary
+    hits (other lights, another diffuse seed; about 200k shadow and 100k diffuse
+    rays sampled), so the timed rays are not the training rays."""
     import torch
+    from realtimeraytracing_amd import RAY_DTYPE, devmem, scenes
     from realtimeraytracing_amd.order import OrderModel
     t0 = time.perf_counter()
     model = OrderModel(ctx, seed=1)
+    n = wl.n_diff
+    tdiff, tshad = devmem.empty(n, RAY_DTYPE), devmem.empty(4 * n, RAY_DTYPE)
+    wl.sc.raygen_diffuse(wl.prim.data_ptr(), wl.phit.data_ptr(), n, TRAIN_DIFFUSE_SEED, scenes.SECONDARY_EPS,
+                         tdiff.data_ptr(), stream)
+    wl.sc.raygen_shadow(wl.prim.data_ptr(), wl.phit.data_ptr(), n, TRAIN_LIGHTS, scenes.SECONDARY_EPS,
+                        tshad.data_ptr(), stream)
     torch.cuda.synchronize()
-    ss, ks = model.log(wl.sc, wl.shad, wl.n_shad, True, stride=max(1, wl.n_shad // 200_000), capacity=1 << 22)
-    sd, kd = model.log(wl.sc, wl.diff, wl.n_diff, False, stride=max(1, wl.n_diff // 100_000), capacity=1 << 22)
+    ss, ks = model.log(wl.sc, tshad, 4 * n, True, stride=max(1, 4 * n // 200_000), capacity=1 << 22)
+    sd, kd = model.log(wl.sc, tdiff, n, False, stride=max(1, n // 100_000), capacity=1 << 22)
+    del tdiff, tshad
     samples = torch.cat([ss, sd]).contiguous()
     losses = model.train(samples, epochs=6, batch=1024, lr=2e-3) if len(samples) else []
     # decisions only for any-hit rays, and only when the logs held any-hit samples
     model.apply(wl.sc, confidence_any=0.5 if ks else 1.0, confidence_closest=1.0)
     torch.cuda.synchronize()
     return {"samples_any": ks, "samples_closest": kd, "epochs": 6,
+            "trained_on": "held-out rays: shadow rays to 4 other lights, diffuse seed 101",
             "loss_first": round(float(losses[0]), 4) if len(losses) else None,
             "loss_last": round(float(losses[-1]), 4) if len(losses) else None,
             "setup_s": round(time.perf_counter() - t0, 2)}

@@ -124,8 +124,12 @@ void rtq_build_params_default(rtq_build_params* p);
  * centroids and Morton codes (305-372), stable (code, index) sort (214-232),
  * PLOC (26-210) and the pre-order flatten (scene.cpp:189-208), then the
  * traversal layout.  Replaces BVH::BVH (bvh.cpp:11-24).  Waits for work queued
- * on the device first: a rebuild rewrites the scene's arrays in place.
- * Temporaries come from the context's scratch, which keeps its largest size. */
+ * on the device first: a rebuild rewrites the scene's arrays in place, and
+ * its node arrays reuse the previous build's memory where it is large enough
+ * (a scene keeps its largest node arrays until rtq_scene_release).  The
+ * 8-wide layout's cost table runs on a second stream of the context beside
+ * the binary layout.  Temporaries come from the context's scratch, which keeps
+ * its largest size. */
 rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params* params);
 
 /* Refit: the built tree's topology kept (PLOC merges, leaf order, binary

@@ -47,16 +47,18 @@ static void release(T*& p) {
     p = nullptr;
 }
 
-// Before a rebuild: the buffers sized by the tree go; those sized by the
-// triangle count (cluster arrays, sorted codes, reference nodes, traversal
-// triangles) and the fixed-size top table stay for the build to overwrite.
+// Before a rebuild: the buffers sized by the tree are let go, their memory
+// kept in s->kept_* for the build to take back when it is large enough; those
+// sized by the triangle count (cluster arrays, sorted codes, reference nodes,
+// traversal triangles) and the fixed-size top table stay for the build to
+// overwrite.
 void reset_build(Scene* s) {
-    release(s->nodes);
-    release(s->node_aux);
-    release(s->wide);
-    release(s->wide_trav);
+    s->nodes = nullptr;
+    s->node_aux = nullptr;
+    s->wide = nullptr;
+    s->wide_trav = nullptr;
     s->wide_ordered = false;
-    release(s->d_passes);
+    s->d_passes = nullptr;
     s->small_from = 0;
     s->n_top = 0;
     s->built = false;
@@ -72,8 +74,19 @@ void reset_build(Scene* s) {
     s->ref_ready = false;
 }
 
+static void release(Kept& k) {
+    if (k.p) (void)hipFree(k.p);
+    k.p = nullptr;
+    k.bytes = 0;
+}
+
 void free_build(Scene* s) {
     reset_build(s);
+    release(s->kept_nodes);
+    release(s->kept_aux);
+    release(s->kept_wide);
+    release(s->kept_wide_trav);
+    release(s->kept_passes);
     release(s->cbox);
     release(s->ctri);
     release(s->cleft);
@@ -303,8 +316,16 @@ rtq_status rtq_create(rtq_ctx** out, int hip_device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(&ctx->pinned, kPinnedBytes, hipHostMallocDefault) != hipSuccess ||
         scratch_alloc(ctx, &ctx->scratch) != RTQ_OK || noalloc_setup(ctx) != RTQ_OK) {
         scratch_free(&ctx->scratch);
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        if (ctx->join) (void)hipEventDestroy(ctx->join);
+        if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return RTQ_EHIP;
@@ -333,10 +354,15 @@ void rtq_destroy(rtq_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
     for (auto& kv : ctx->scenes) free_scene(kv.second);
     ctx->scenes.clear();
     scratch_free(&ctx->scratch);
     if (ctx->arena.base) (void)hipFree(ctx->arena.base);
+    (void)hipHostFree(ctx->pinned);
+    (void)hipEventDestroy(ctx->join);
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

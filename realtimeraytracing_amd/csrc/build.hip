@@ -1126,9 +1126,15 @@ rtq_status ploc_fused(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint3
         hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(kTailThreads), 0, st, state, radius, err, tail_log, tail_count,
                            lists, cl);
         RTQ_HIP(ctx, hipGetLastError());
+        // the head block and the first tail log entries in one read (entries
+        // past the tail's pass count are not used)
+        constexpr uint32_t kTailPeek = 128;
         uint32_t h_head[kHeadWords];
-        RTQ_HIP(ctx, hipMemcpyAsync(h_head, state, sizeof h_head, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        uint4 h_peek[kTailPeek];
+        HostReads rd(ctx, st);
+        RTQ_HIP(ctx, rd.add(h_head, state, sizeof h_head));
+        RTQ_HIP(ctx, rd.add(h_peek, tail_log, sizeof h_peek));
+        RTQ_HIP(ctx, rd.wait());
         const uint32_t* h_state = h_head;
         const uint32_t* h_err = h_head + 8;
         uint4 h_log[kPlocBatch];
@@ -1147,8 +1153,12 @@ rtq_status ploc_fused(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint3
         if (nt > kTailMax) return fail(ctx, RTQ_EINTERNAL, "PLOC tail ran %u passes", nt);
         if (nt) {
             h_tail.resize(nt);
-            RTQ_HIP(ctx, hipMemcpyAsync(h_tail.data(), tail_log, sizeof(uint4) * nt, hipMemcpyDeviceToHost, st));
-            RTQ_HIP(ctx, hipStreamSynchronize(st));
+            if (nt <= kTailPeek) {
+                memcpy(h_tail.data(), h_peek, sizeof(uint4) * nt);
+            } else {
+                RTQ_HIP(ctx, rd.add(h_tail.data(), tail_log, sizeof(uint4) * nt));
+                RTQ_HIP(ctx, rd.wait());
+            }
             for (const uint4& e : h_tail) passes.emplace_back(e.x, e.y);
         }
         if (h_state[0] >= iter && h_state[0] > 1)
@@ -1205,9 +1215,10 @@ rtq_status ploc_split(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint3
         }
         uint4 h_log[kPlocBatch];
         uint32_t h_state[2];
-        RTQ_HIP(ctx, hipMemcpyAsync(h_log, pass_log, sizeof h_log, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipMemcpyAsync(h_state, state, sizeof h_state, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipStreamSynchronize(st));
+        HostReads rd(ctx, st);
+        RTQ_HIP(ctx, rd.add(h_log, pass_log, sizeof h_log));
+        RTQ_HIP(ctx, rd.add(h_state, state, sizeof h_state));
+        RTQ_HIP(ctx, rd.wait());
         for (uint32_t k = 0; k < kPlocBatch; ++k) {
             const uint4 e = h_log[k];  // (first merge id, merges, clusters at the pass)
             if (e.z <= 1) break;
@@ -1227,7 +1238,41 @@ rtq_status ploc_split(rtq_ctx* ctx, Scene* s, uint32_t radius, Temps& tmp, uint3
 // scene extent, the binary traversal nodes and (node_width 8) the compressed
 // wide nodes.  Buffers that exist already are written in place (a refit keeps
 // their addresses, so captured batch graphs stay valid).
-rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipStream_t st) {
+// Work queued on the context's second stream: fork(st) makes it wait for
+// what st has queued, join(st) makes st wait for it.  Forked work that is
+// never joined (an early return) is waited for on the host when the guard
+// goes out of scope, before the temporaries it uses are given back.
+class SideWork {
+public:
+    explicit SideWork(rtq_ctx* ctx) : ctx_(ctx) {}
+    SideWork(const SideWork&) = delete;
+    SideWork& operator=(const SideWork&) = delete;
+    ~SideWork() {
+        if (forked_ && !joined_) (void)hipStreamSynchronize(ctx_->side);
+    }
+    hipError_t fork(hipStream_t st) {
+        hipError_t e = hipEventRecord(ctx_->fork, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx_->side, ctx_->fork, 0);
+        forked_ = true;  // even on an error: the destructor then waits for the side stream
+        return e;
+    }
+    hipError_t join(hipStream_t st) {
+        if (!forked_ || joined_) return hipSuccess;
+        hipError_t e = hipEventRecord(ctx_->join, ctx_->side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ctx_->join, 0);
+        if (e == hipSuccess) joined_ = true;
+        return e;
+    }
+
+private:
+    rtq_ctx* ctx_;
+    bool forked_ = false, joined_ = false;
+};
+
+// flag_dev (optional): one more device word to read back with the layout's
+// own host read, into *flag_host.
+rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipStream_t st,
+                         const uint32_t* flag_dev = nullptr, uint32_t* flag_host = nullptr) {
     const uint32_t n = s->n_tris;
     const uint32_t nc = s->n_clusters;
     Temps tmp(ctx);
@@ -1246,6 +1291,26 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     RTQ_HIP(ctx, tmp.get(&bidx, nc));
     RTQ_HIP(ctx, tmp.get(&ntrav, 1));
     const uint32_t root = nc - 1;
+    // The 8-wide collapse's cost table reads the clusters only: it runs on the
+    // second stream beside the placement and the binary nodes, its launches
+    // queued between the placement's so that both streams start at once.
+    float* wcost = nullptr;
+    uint32_t* wchoice = nullptr;
+    SideWork side(ctx);
+    const uint32_t dp_steps = s->params.node_width == 8 ? wide_cost_steps(s) : 0u;
+    uint32_t dp_next = 0;
+    auto queue_cost = [&](uint32_t upto) -> rtq_status {
+        upto = std::min(upto, dp_steps);
+        if (dp_next >= upto) return RTQ_OK;
+        const rtq_status r = wide_cost_table(ctx, s, wcost, wchoice, dp_next, upto, ctx->side);
+        dp_next = upto;
+        return r;
+    };
+    if (dp_steps) {
+        RTQ_HIP(ctx, tmp.get(&wcost, (size_t)(nc - n) * 8));
+        RTQ_HIP(ctx, tmp.get(&wchoice, nc - n));
+        RTQ_HIP(ctx, side.fork(st));
+    }
     hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
     RTQ_HIP(ctx, hipGetLastError());
     // the small passes at the top of the tree in one launch, then one launch
@@ -1256,23 +1321,40 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
                            s->cleft, s->cright, s->ccount, pos, fl, dep);
         RTQ_HIP(ctx, hipGetLastError());
     }
+    rtq_status rs = queue_cost(2);
+    if (rs != RTQ_OK) return rs;
     for (uint32_t k = s->small_from; k > 0; --k) {
         const auto& pr = s->passes[k - 1];
         hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
                            s->cleft, s->cright, s->ccount, pos, fl, dep);
         RTQ_HIP(ctx, hipGetLastError());
+        rs = queue_cost(dp_next + 1);
+        if (rs != RTQ_OK) return rs;
     }
     hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->ctri, s->cleft, fl,
                        dep, s->tri_world, s->tris, bdep);
     RTQ_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, bdep, blocks_for(nc), maxd);
     RTQ_HIP(ctx, hipGetLastError());
+    if (n > 1) {
+        hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, s->params.max_leaf_size,
+                           s->cleft, s->ccount, pos, flag);
+        RTQ_HIP(ctx, hipGetLastError());
+        rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
+        if (rs != RTQ_OK) return rs;
+    }
 
+    // one host read: scene bounds (box pad), tree depth, traversal node count
     rtq_aabb h_bounds[2];
-    uint32_t h_depth = 0;
-    RTQ_HIP(ctx, hipMemcpyAsync(h_bounds, d_bounds, sizeof h_bounds, hipMemcpyDeviceToHost, st));
-    RTQ_HIP(ctx, hipMemcpyAsync(&h_depth, maxd, sizeof h_depth, hipMemcpyDeviceToHost, st));
-    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    uint32_t h_depth = 0, h_ntrav = 1;
+    HostReads rd(ctx, st);
+    RTQ_HIP(ctx, rd.add(h_bounds, d_bounds, sizeof h_bounds));
+    RTQ_HIP(ctx, rd.add(&h_depth, maxd, sizeof h_depth));
+    if (n > 1) RTQ_HIP(ctx, rd.add(&h_ntrav, ntrav, sizeof h_ntrav));
+    if (flag_dev) RTQ_HIP(ctx, rd.add(flag_host, flag_dev, sizeof(uint32_t)));
+    rs = queue_cost(dp_steps);
+    if (rs != RTQ_OK) return rs;
+    RTQ_HIP(ctx, rd.wait());
     float extent = 0.0f;
     for (int k = 0; k < 3; ++k)
         extent = std::max(extent, std::max(std::fabs(h_bounds[0].min[k]), std::fabs(h_bounds[0].max[k])));
@@ -1280,36 +1362,26 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     s->depth = h_depth;
 
     const uint32_t max_leaf = s->params.max_leaf_size;
+    if (s->nodes && h_ntrav != s->n_trav)
+        return fail(ctx, RTQ_EINTERNAL, "refit: %u traversal nodes, the tree has %u", h_ntrav, s->n_trav);
+    s->n_trav = h_ntrav;
+    if (!s->nodes) RTQ_HIP(ctx, take(s->kept_nodes, &s->nodes, 4ull * h_ntrav));
+    if (!s->node_aux) RTQ_HIP(ctx, take(s->kept_aux, &s->node_aux, h_ntrav));
     if (n == 1) {
-        s->n_trav = 1;
-        if (!s->nodes) RTQ_HIP(ctx, dev_alloc(&s->nodes, 4));
-        if (!s->node_aux) RTQ_HIP(ctx, dev_alloc(&s->node_aux, 1));
         hipLaunchKernelGGL(k_write_single, dim3(1), dim3(1), 0, st, s->cbox, s->box_pad, s->nodes, s->node_aux);
         RTQ_HIP(ctx, hipGetLastError());
     } else {
-        hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
-                           s->cleft, s->ccount, pos, flag);
-        RTQ_HIP(ctx, hipGetLastError());
-        rtq_status rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
-        if (rs != RTQ_OK) return rs;
-        uint32_t h_ntrav = 0;
-        RTQ_HIP(ctx, hipMemcpyAsync(&h_ntrav, ntrav, sizeof h_ntrav, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipStreamSynchronize(st));
-        if (s->nodes && h_ntrav != s->n_trav)
-            return fail(ctx, RTQ_EINTERNAL, "refit: %u traversal nodes, the tree has %u", h_ntrav, s->n_trav);
-        s->n_trav = h_ntrav;
-        if (!s->nodes) RTQ_HIP(ctx, dev_alloc(&s->nodes, 4ull * h_ntrav));
-        if (!s->node_aux) RTQ_HIP(ctx, dev_alloc(&s->node_aux, h_ntrav));
         hipLaunchKernelGGL(k_write_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, max_leaf,
                            s->box_pad, s->cbox, s->cleft, s->cright, s->ccount, pos, fl, bidx, dep, s->nodes,
                            s->node_aux);
         RTQ_HIP(ctx, hipGetLastError());
     }
     if (s->params.node_width == 8) {
-        rtq_status rs = build_wide(ctx, s, fl, pos, n == 1 ? nullptr : bidx, st);
+        RTQ_HIP(ctx, side.join(st));
+        rs = build_wide(ctx, s, wchoice, fl, pos, n == 1 ? nullptr : bidx, st);
         if (rs != RTQ_OK) return rs;
     }
-    rtq_status rs = write_top(ctx, s, st);
+    rs = write_top(ctx, s, st);
     if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipStreamSynchronize(st));
     return RTQ_OK;
@@ -1343,12 +1415,13 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     // size the context's scratch for this build's temporaries up front, so a
     // first build does not allocate them one by one.  Alive together, per
     // triangle: sort keys and values and the PLOC lists 64 B, placement 32 B;
-    // with 8-wide nodes also the collapse cost table 36 B and the collapse
-    // buffers 52 B per binary traversal node (one per triangle with leaves of
-    // one triangle); the fused PLOC passes' tile status words (16 passes x
-    // 8 B per tile of 480) and each placement workgroup's deepest leaf, under
-    // n / 2 B; 1 MiB for scans and 256-B rounding
-    const size_t per_tri = prm.node_width == 8 ? 64 + 32 + 36 + 52 : 64 + 32;
+    // with 8-wide nodes also the collapse cost table 36 B and the collapse's
+    // two level lists 8 B per binary traversal node (one per triangle with
+    // leaves of one triangle); the fused PLOC passes' tile status words (16
+    // passes x 8 B per tile of 480), each placement workgroup's deepest leaf
+    // and the collapse's tile status words, under n / 2 B; 1 MiB for scans and
+    // 256-B rounding
+    const size_t per_tri = prm.node_width == 8 ? 64 + 32 + 36 + 8 : 64 + 32;
     ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + n / 2 + ((size_t)1 << 20));
     rtq_status rs = arena_reserve(ctx);
     if (rs != RTQ_OK) return rs;
@@ -1420,19 +1493,18 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
         s->h_passes.resize(s->ploc_passes);
         for (uint32_t k = 0; k < s->ploc_passes; ++k)
             s->h_passes[k] = make_uint2(s->passes[k].first, s->passes[k].second);
-        RTQ_HIP(ctx, dev_alloc(&s->d_passes, s->ploc_passes));
+        RTQ_HIP(ctx, take(s->kept_passes, &s->d_passes, s->ploc_passes));
         RTQ_HIP(ctx, hipMemcpyAsync(s->d_passes, s->h_passes.data(), sizeof(uint2) * s->ploc_passes,
                                     hipMemcpyHostToDevice, st));
     }
 
     // 3-4. node arrays of every layout
-    rs = write_layouts(ctx, s, d_bounds, st);
-    if (rs != RTQ_OK) return rs;
-    RTQ_HIP(ctx, hipEventRecord(ev1, st));
     uint32_t h_sort_err = 0;
-    RTQ_HIP(ctx, hipMemcpyAsync(&h_sort_err, sort_err, sizeof h_sort_err, hipMemcpyDeviceToHost, st));
-    RTQ_HIP(ctx, hipStreamSynchronize(st));
+    rs = write_layouts(ctx, s, d_bounds, st, sort_err, &h_sort_err);
+    if (rs != RTQ_OK) return rs;
     if (h_sort_err) return fail(ctx, RTQ_EINTERNAL, "build: a radix sort pass's look-back gave up");
+    RTQ_HIP(ctx, hipEventRecord(ev1, st));
+    RTQ_HIP(ctx, hipEventSynchronize(ev1));
     float ms = 0.0f;
     RTQ_HIP(ctx, hipEventElapsedTime(&ms, ev0, ev1));
     s->build_ms = ms;
@@ -1453,8 +1525,9 @@ rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st) {
     hipLaunchKernelGGL(k_top_nodes, dim3(1), dim3(64), 0, st, s->nodes, s->top, d_n);
     hipError_t e = hipGetLastError();
     uint32_t n = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&n, d_n, sizeof n, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    HostReads rd(ctx, st);
+    if (e == hipSuccess) e = rd.add(&n, d_n, sizeof n);
+    if (e == hipSuccess) e = rd.wait();
     if (e != hipSuccess) return hip_fail(ctx, e, "write_top");
     s->n_top = n;
     return RTQ_OK;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <string>
 #include <utility>
@@ -66,6 +67,14 @@ struct DevModel {
 };
 static_assert(sizeof(DevModel) == 68, "MeshModelGPU is 68 B");
 
+// A device buffer that outlives the build it was allocated for: the next
+// build of the scene takes it back when it is large enough (take()), so a
+// rebuild does not free and allocate its node arrays again.
+struct Kept {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
 struct Scene {
     uint32_t n_tris = 0;
     uint32_t n_models = 0;
@@ -111,6 +120,9 @@ struct Scene {
     uint2* d_passes = nullptr;
     std::vector<uint2> h_passes;
     uint32_t small_from = 0;
+    // the memory behind nodes, node_aux, wide, wide_trav and d_passes, kept
+    // from build to build
+    Kept kept_nodes, kept_aux, kept_wide, kept_wide_trav, kept_passes;
     // vertices changed since the last build or refit (rtq_scene_update)
     bool stale = false;
     float box_pad = 0.0f;
@@ -154,6 +166,13 @@ struct rtq_ctx {
     int device = 0;
     int n_cu = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for build work that runs beside the
+    // main stream's, forked and joined with the two events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    // pinned host memory (rtq::kPinnedBytes) for the build's small device-to-
+    // host reads (rtq::HostReads)
+    void* pinned = nullptr;
     std::string err;
     std::map<uint32_t, rtq::Scene*> scenes;
     uint32_t next_scene = 1;
@@ -178,6 +197,50 @@ struct rtq_ctx {
     std::map<uint64_t, rtq_launch_params> tuned;
 };
 
+namespace rtq {
+constexpr size_t kPinnedBytes = 128 << 10;
+
+// Device-to-host reads staged through the context's pinned buffer (a copy to
+// pageable memory is staged and waited for one by one): add() queues a copy
+// on st into the next free bytes, wait() synchronises st once and hands every
+// read to its destination.  One HostReads at a time per context.
+class HostReads {
+public:
+    HostReads(rtq_ctx* ctx, hipStream_t st) : ctx_(ctx), st_(st) {}
+    hipError_t add(void* dst, const void* src, size_t bytes) {
+        const size_t off = (used_ + 15) & ~size_t(15);
+        if (n_ == kMaxReads || off + bytes > kPinnedBytes) return hipErrorInvalidValue;
+        const hipError_t e =
+            hipMemcpyAsync(static_cast<char*>(ctx_->pinned) + off, src, bytes, hipMemcpyDeviceToHost, st_);
+        if (e != hipSuccess) return e;
+        reads_[n_++] = Read{dst, off, bytes};
+        used_ = off + bytes;
+        return hipSuccess;
+    }
+    hipError_t wait() {
+        const hipError_t e = hipStreamSynchronize(st_);
+        if (e != hipSuccess) return e;
+        for (int i = 0; i < n_; ++i)
+            std::memcpy(reads_[i].dst, static_cast<const char*>(ctx_->pinned) + reads_[i].off, reads_[i].bytes);
+        n_ = 0;
+        used_ = 0;
+        return hipSuccess;
+    }
+
+private:
+    struct Read {
+        void* dst;
+        size_t off, bytes;
+    };
+    static constexpr int kMaxReads = 8;
+    rtq_ctx* ctx_;
+    hipStream_t st_;
+    Read reads_[kMaxReads] = {};
+    int n_ = 0;
+    size_t used_ = 0;
+};
+}  // namespace rtq
+
 struct rtq_batch {
     rtq_ctx* ctx = nullptr;
     std::vector<rtq_batch_item> items;
@@ -200,6 +263,28 @@ void reset_build(Scene* s);
 template <typename T>
 hipError_t dev_alloc(T** p, size_t count) {
     return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+}
+
+// *p = count elements of k's memory, reallocated only when k holds fewer bytes.
+template <typename T>
+hipError_t take(Kept& k, T** p, size_t count) {
+    const size_t bytes = sizeof(T) * (count ? count : 1);
+    if (k.bytes < bytes) {
+        if (k.p) {
+            const hipError_t e = hipFree(k.p);
+            k.p = nullptr;
+            k.bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        const hipError_t e = hipMalloc(&k.p, bytes);
+        if (e != hipSuccess) {
+            k.p = nullptr;
+            return e;
+        }
+        k.bytes = bytes;
+    }
+    *p = static_cast<T*>(k.p);
+    return hipSuccess;
 }
 
 // Temporaries of one scope, taken from the top of the context's arena and
@@ -282,8 +367,17 @@ void scratch_free(TraceScratch* sc);
 // Compressed 8-wide nodes from the cluster arrays (fl: first leaf rank per cluster).
 // pos, bidx: pre-order position of every cluster and binary traversal index
 // of every position (null for a one-triangle scene, which has no binary node).
-rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, const uint32_t* pos, const uint32_t* bidx,
-                      hipStream_t st);
+// choice: the collapse's cost-table decisions (wide_cost_table), null for a
+// one-triangle scene.
+rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint32_t* fl, const uint32_t* pos,
+                      const uint32_t* bidx, hipStream_t st);
+// The SAH cost table of the 8-wide collapse (cost: 8 floats, choice: one word
+// per internal cluster), bottom-up over the PLOC passes in wide_cost_steps(s)
+// launches; wide_cost_table queues launches [first, last) of them.  It reads
+// the cluster arrays only, so it can run beside the binary layout's placement.
+uint32_t wide_cost_steps(const Scene* s);
+rtq_status wide_cost_table(rtq_ctx* ctx, const Scene* s, float* cost, uint32_t* choice, uint32_t first,
+                           uint32_t last, hipStream_t st);
 // Learned first-child tables of the 8-wide nodes from the binary nodes' order
 // words (learned.hip); clear = zero them.
 rtq_status write_wide_order(rtq_ctx* ctx, Scene* s, bool clear, hipStream_t st);

@@ -29,9 +29,10 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWidth = 8;
 
-__device__ __forceinline__ bool expandable(uint32_t c, uint32_t max_leaf, const uint32_t* cleft,
-                                           const uint32_t* ccount) {
-    return cleft[c] != kNone && ccount[c] > max_leaf;
+// a child that becomes an 8-wide node of the next level: a merged cluster
+// (ids from nt on) of more than max_leaf triangles
+__device__ __forceinline__ bool expandable(uint32_t c, uint32_t max_leaf, uint32_t nt, const uint32_t* ccount) {
+    return c >= nt && ccount[c] > max_leaf;
 }
 
 __device__ __forceinline__ float box_area(const rtq_aabb& b) {
@@ -125,60 +126,39 @@ __global__ __launch_bounds__(kDpSmallThreads) void k_dp_small(const uint2* __res
     }
 }
 
-// A level's node count and first node index are device words (lvl[0], lvl[1])
-// so several levels queue without a host read; the grid covers an upper bound
-// `ub` of the count.
-__global__ __launch_bounds__(kThreads) void k_wide_collapse(uint32_t ub, const uint32_t* __restrict__ lvl,
-                                                            const uint32_t* __restrict__ items,
-                                                            uint32_t max_leaf, const uint32_t* __restrict__ cleft,
-                                                            const uint32_t* __restrict__ cright,
-                                                            const uint32_t* __restrict__ ccount,
-                                                            const rtq_aabb* __restrict__ cbox,
-                                                            uint32_t nt, const uint32_t* __restrict__ choice,
-                                                            uint32_t* __restrict__ children,
-                                                            uint32_t* __restrict__ n_children,
-                                                            uint32_t* __restrict__ n_expand) {
-    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= ub) return;
-    if (k >= lvl[0]) {
-        n_expand[k] = 0;   // the scan covers the bound
-        return;
-    }
-    const uint32_t c = items[k];
-    uint32_t ch[kWidth];
+// The slots of node cluster c as the cost table splits them, children left
+// to right; returns their count.  Clusters below nt are the single triangles
+// (no children), so a step's three loads do not wait for one another.
+__device__ __forceinline__ uint32_t collapse_node(uint32_t c, const uint32_t* __restrict__ cleft,
+                                                  const uint32_t* __restrict__ cright, uint32_t nt,
+                                                  const uint32_t* __restrict__ choice, uint32_t* ch) {
     uint32_t n = 0;
-    if (cleft[c] == kNone) {
+    if (c < nt) {
         ch[n++] = c;  // single-triangle scene: the root itself is the only (leaf) child
-    } else {
-        // the slots as the cost table splits them, children left to right
-        const uint32_t kn = ((choice[c - nt] >> 21) & 7u) + 1u;
-        uint32_t sx[kWidth], sj[kWidth];
-        int sp = 0;
-        sx[sp] = cright[c]; sj[sp++] = kWidth - kn;
-        sx[sp] = cleft[c]; sj[sp++] = kn;
-        while (sp > 0) {
-            --sp;
-            const uint32_t x = sx[sp], j = sj[sp];
-            if (j == 1 || cleft[x] == kNone) {
-                ch[n++] = x;
-                continue;
-            }
-            const uint32_t pick = (choice[x - nt] >> (3 * (j - 2))) & 7u;
-            if (pick == 0) {
-                sx[sp] = x; sj[sp++] = j - 1;
-            } else {
-                sx[sp] = cright[x]; sj[sp++] = j - pick;
-                sx[sp] = cleft[x]; sj[sp++] = pick;
-            }
+        return n;
+    }
+    const uint32_t kn = ((choice[c - nt] >> 21) & 7u) + 1u;
+    uint32_t sx[kWidth], sj[kWidth];
+    int sp = 0;
+    sx[sp] = cright[c]; sj[sp++] = kWidth - kn;
+    sx[sp] = cleft[c]; sj[sp++] = kn;
+    while (sp > 0) {
+        --sp;
+        const uint32_t x = sx[sp], j = sj[sp];
+        if (j == 1 || x < nt) {
+            ch[n++] = x;
+            continue;
+        }
+        const uint32_t cw = choice[x - nt], l = cleft[x], r = cright[x];
+        const uint32_t pick = (cw >> (3 * (j - 2))) & 7u;
+        if (pick == 0) {
+            sx[sp] = x; sj[sp++] = j - 1;
+        } else {
+            sx[sp] = r; sj[sp++] = j - pick;
+            sx[sp] = l; sj[sp++] = pick;
         }
     }
-    uint32_t ne = 0;
-    for (uint32_t i = 0; i < kWidth; ++i) {
-        children[(size_t)k * kWidth + i] = i < n ? ch[i] : kNone;
-        if (i < n && expandable(ch[i], max_leaf, cleft, ccount)) ++ne;
-    }
-    n_children[k] = n;
-    n_expand[k] = ne;
+    return n;
 }
 
 __device__ int exponent_for(double extent) {
@@ -190,37 +170,174 @@ __device__ int exponent_for(double extent) {
     return e;
 }
 
-__global__ __launch_bounds__(kThreads) void k_wide_write(
-    uint32_t ub, const uint32_t* __restrict__ lvl, uint32_t cap, uint32_t* __restrict__ err, uint32_t max_leaf,
-    float pad,
-    const uint32_t* __restrict__ items, const uint32_t* __restrict__ children, const uint32_t* __restrict__ n_children,
-    const uint32_t* __restrict__ child_offs, const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ ccount,
-    const rtq_aabb* __restrict__ cbox, const uint32_t* __restrict__ fl, const uint32_t* __restrict__ pos,
-    const uint32_t* __restrict__ bidx, float4* __restrict__ nodes, uint32_t* __restrict__ wide_trav,
-    uint32_t* __restrict__ next_items) {
-    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    const uint32_t n_items = lvl[0], level_base = lvl[1], next_base = level_base + n_items;
-    if (k >= ub || k >= n_items) return;
+// Tile status of the level scan: value in the low 32 bits, state on top.
+constexpr uint64_t kLvAgg = 1ull << 62;     // the tile's own count
+constexpr uint64_t kLvPre = 2ull << 62;     // the counts through the tile
+constexpr uint64_t kLvPoison = 3ull << 62;  // a wait at or before the tile gave up
+constexpr uint64_t kLvState = 3ull << 62;
+constexpr uint32_t kLvSpinLimit = 1u << 24;
+enum : uint32_t { kWideErrOverflow = 1, kWideErrScan = 2 };
+
+struct LevelArgs {
+    uint32_t* lv;              // this level's words: count, first node, first status word, tile ticket
+    uint64_t* status;          // tile status words, n_status of them, zero before the first level
+    uint32_t n_status;
+    uint32_t cap;              // node capacity
+    uint32_t* err;
+    uint32_t max_leaf;
+    uint32_t nt;
+    float pad;
+    const uint32_t* items;     // this level's node clusters
+    uint32_t* next_items;      // the next level's
+    const uint32_t* cleft;
+    const uint32_t* cright;
+    const uint32_t* ccount;
+    const rtq_aabb* cbox;
+    const uint32_t* choice;
+    const uint32_t* fl;
+    const uint32_t* pos;
+    const uint32_t* bidx;
+    float4* nodes;
+    uint32_t* wide_trav;
+};
+
+// One level of the collapse in one launch.  Each thread takes one node of the
+// level and picks its slots; the next level's nodes are numbered by an
+// exclusive scan of the nodes' expandable children in node order, across
+// workgroups by decoupled look-back (tiles numbered in the order workgroups
+// start, so every tile waited on is running; the wait is bounded and a tile
+// that gives up poisons its status and sets an error bit); then the thread
+// writes its node.  The last tile writes the next level's words (lv[4..7]).
+// The grid covers an upper bound of the level's count; workgroups past the
+// count, and every workgroup of an empty level, return at once.
+__global__ __launch_bounds__(kThreads) void k_wide_level(LevelArgs a) {
+    __shared__ uint32_t s_tile, s_excl, s_bad, s_w[kThreads / 64];
+    const uint32_t count = a.lv[0];
+    const uint32_t tiles = (count + kThreads - 1) / kThreads;
+    if (blockIdx.x >= tiles) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(a.lv + 3, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t k = tile * kThreads + tid;
+    const uint32_t level_base = a.lv[1], sbase = a.lv[2];
+
+    uint32_t ch[kWidth];
+    uint32_t n = 0, ne = 0, expand = 0;
+    if (k < count) {
+        n = collapse_node(a.items[k], a.cleft, a.cright, a.nt, a.choice, ch);
+        for (uint32_t i = 0; i < n; ++i)
+            if (expandable(ch[i], a.max_leaf, a.nt, a.ccount)) {
+                expand |= 1u << i;
+                ++ne;
+            }
+    }
+    // exclusive sum of ne over the workgroup
+    uint32_t x = ne;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if ((int)lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kThreads / 64; ++w) {
+        if (w < wave) wave_off += s_w[w];
+        total += s_w[w];
+    }
+    const uint32_t local = wave_off + x - ne;
+
+    // counts before this tile: decoupled look-back by wave 0
+    if (wave == 0) {
+        bool failed = (uint64_t)sbase + tiles > a.n_status;
+        uint32_t excl = 0;
+        if (!failed) {
+            uint64_t* const st = a.status + sbase;
+            if (tile == 0) {
+                if (lane == 0) __hip_atomic_store(&st[0], kLvPre | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                if (lane == 0)
+                    __hip_atomic_store(&st[tile], kLvAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int64_t top = (int64_t)tile - 1;
+                uint32_t spins = 0;
+                for (;;) {
+                    const int64_t idx = top - (int64_t)lane;  // lane 0 = nearest tile
+                    uint64_t w = kLvPre;                       // before tile 0: an empty prefix
+                    if (idx >= 0) w = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    while (__ballot((w & kLvState) == 0) != 0) {
+                        if (++spins > kLvSpinLimit) {
+                            failed = true;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        if ((w & kLvState) == 0)
+                            w = __hip_atomic_load(&st[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    if (failed) break;
+                    // nearest tile with counts through it (a prefix, or poison)
+                    const unsigned long long pre = __ballot((w & kLvState) >= kLvPre);
+                    const int stop = pre ? __ffsll((long long)pre) - 1 : 63;
+                    if (pre && __shfl((uint32_t)(w >> 62), stop, 64) == 3u) {
+                        failed = true;
+                        break;
+                    }
+                    uint32_t c = (int)lane <= stop ? (uint32_t)w : 0u;
+#pragma unroll
+                    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+                    excl += c;
+                    if (pre) break;
+                    top -= 64;
+                }
+                if (lane == 0)
+                    __hip_atomic_store(&st[tile], failed ? kLvPoison : (kLvPre | (uint64_t)(excl + total)),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (lane == 0) {
+            s_excl = excl;
+            s_bad = failed ? 1u : 0u;
+            if (failed) atomicOr(a.err, kWideErrScan);
+        }
+    }
+    __syncthreads();
+    if (s_bad) return;
+    const uint32_t excl = s_excl;
+    const uint32_t next_base = level_base + count;
+    if (tile == tiles - 1 && tid == 0) {
+        uint32_t next_count = excl + total;
+        if ((uint64_t)next_base + next_count > a.cap) {
+            atomicOr(a.err, kWideErrOverflow);   // more wide nodes than binary ones: a broken invariant
+            next_count = 0;
+        }
+        a.lv[4] = next_count;
+        a.lv[5] = next_base;
+        a.lv[6] = sbase + tiles;
+        a.lv[7] = 0u;
+    }
+    if (k >= count) return;
+
+    // the node: children boxes quantised outward against the node's origin
     // the binary traversal node of this node's root cluster (learned order)
-    wide_trav[level_base + k] = bidx ? bidx[pos[items[k]]] : kNone;
-    const uint32_t n = n_children[k];
+    a.wide_trav[level_base + k] = a.bidx ? a.bidx[a.pos[a.items[k]]] : kNone;
     float lo[kWidth][3], hi[kWidth][3];
     float org[3] = {INFINITY, INFINITY, INFINITY};
     double top[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < n; ++i) {
-        const rtq_aabb b = cbox[children[(size_t)k * kWidth + i]];
-        for (int a = 0; a < 3; ++a) {
-            lo[i][a] = b.min[a] - pad;
-            hi[i][a] = b.max[a] + pad;
-            org[a] = fminf(org[a], lo[i][a]);
-            top[a] = fmax(top[a], (double)hi[i][a]);
+        const rtq_aabb b = a.cbox[ch[i]];
+        for (int q = 0; q < 3; ++q) {
+            lo[i][q] = b.min[q] - a.pad;
+            hi[i][q] = b.max[q] + a.pad;
+            org[q] = fminf(org[q], lo[i][q]);
+            top[q] = fmax(top[q], (double)hi[i][q]);
         }
     }
     int ex[3];
     double scale[3];
-    for (int a = 0; a < 3; ++a) {
-        ex[a] = exponent_for(top[a] - (double)org[a]);
-        scale[a] = ldexp(1.0, ex[a]);
+    for (int q = 0; q < 3; ++q) {
+        ex[q] = exponent_for(top[q] - (double)org[q]);
+        scale[q] = ldexp(1.0, ex[q]);
     }
     uint32_t qlo[3][2] = {{0, 0}, {0, 0}, {0, 0}}, qhi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     uint32_t ref[kWidth];
@@ -229,31 +346,31 @@ __global__ __launch_bounds__(kThreads) void k_wide_write(
         ref[i] = 0u;
         if (i >= n) continue;
         valid |= 1u << i;
-        for (int a = 0; a < 3; ++a) {
+        for (int q = 0; q < 3; ++q) {
             // exact: float differences and division by a power of two are exact in double
-            double l = floor(((double)lo[i][a] - (double)org[a]) / scale[a]);
-            double h = ceil(((double)hi[i][a] - (double)org[a]) / scale[a]);
+            double l = floor(((double)lo[i][q] - (double)org[q]) / scale[q]);
+            double h = ceil(((double)hi[i][q] - (double)org[q]) / scale[q]);
             l = fmin(fmax(l, 0.0), 255.0);
             h = fmin(fmax(h, 0.0), 255.0);
-            qlo[a][i >> 2] |= (uint32_t)l << (8 * (i & 3));
-            qhi[a][i >> 2] |= (uint32_t)h << (8 * (i & 3));
+            qlo[q][i >> 2] |= (uint32_t)l << (8 * (i & 3));
+            qhi[q][i >> 2] |= (uint32_t)h << (8 * (i & 3));
         }
-        const uint32_t c = children[(size_t)k * kWidth + i];
-        if (expandable(c, max_leaf, cleft, ccount)) {
-            const uint32_t slot = child_offs[k] + rank++;
+        const uint32_t c = ch[i];
+        if ((expand >> i) & 1u) {
+            const uint32_t slot = excl + local + rank++;
             ref[i] = next_base + slot;
-            if (next_base + slot < cap) {
-                next_items[slot] = c;
+            if ((uint64_t)next_base + slot < a.cap) {
+                a.next_items[slot] = c;
             } else {
-                atomicOr(err, 1u);   // more wide nodes than binary ones: a broken invariant
+                atomicOr(a.err, kWideErrOverflow);
             }
         } else {
-            ref[i] = leaf_ref(fl[c], ccount[c]);
+            ref[i] = leaf_ref(a.fl[c], a.ccount[c]);
         }
     }
     const uint32_t meta = (uint32_t)(ex[0] + 127) | ((uint32_t)(ex[1] + 127) << 8) |
                           ((uint32_t)(ex[2] + 127) << 16) | (valid << 24);
-    float4* o = nodes + 8ull * (level_base + k);
+    float4* o = a.nodes + 8ull * (level_base + k);
     o[0] = make_float4(org[0], org[1], org[2], __uint_as_float(meta));
     o[1] = make_float4(__uint_as_float(qlo[0][0]), __uint_as_float(qlo[0][1]), __uint_as_float(qhi[0][0]),
                        __uint_as_float(qhi[0][1]));
@@ -271,109 +388,119 @@ __global__ __launch_bounds__(kThreads) void k_wide_write(
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
-// The next level's count (the scan total) and first node index; a level that
-// would pass the node capacity ends the tree with an error.
-__global__ void k_level_next(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ total, uint32_t cap,
-                             uint32_t* __restrict__ err, uint32_t* __restrict__ next) {
-    const uint32_t base = lvl[1] + lvl[0];
-    uint32_t count = *total;
-    if ((uint64_t)base + count > cap) {
-        *err = 1u;
-        count = 0;
-    }
-    next[0] = count;
-    next[1] = base;
-}
-
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
 
 }  // namespace
 
+uint32_t wide_cost_steps(const Scene* s) {
+    if (s->n_clusters <= s->n_tris) return 0;
+    return s->small_from + (s->small_from < (uint32_t)s->passes.size() ? 1u : 0u);
+}
+
+rtq_status wide_cost_table(rtq_ctx* ctx, const Scene* s, float* cost, uint32_t* choice, uint32_t first,
+                           uint32_t last, hipStream_t st) {
+    const uint32_t nt = s->n_tris, max_leaf = s->params.max_leaf_size;
+    const uint32_t n_passes = (uint32_t)s->passes.size();
+    for (uint32_t k = first; k < last; ++k) {
+        if (k < s->small_from) {
+            const auto& pr = s->passes[k];
+            hipLaunchKernelGGL(k_dp_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
+                               nt, max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
+        } else {
+            hipLaunchKernelGGL(k_dp_small, dim3(1), dim3(kDpSmallThreads), 0, st, s->d_passes, s->small_from,
+                               n_passes, nt, max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
+        }
+        RTQ_HIP(ctx, hipGetLastError());
+    }
+    return RTQ_OK;
+}
+
 // Builds s->wide (s->n_wide nodes, s->wide_depth levels) from the cluster
-// arrays; fl = first leaf rank of every cluster (device).
-rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* fl, const uint32_t* pos, const uint32_t* bidx,
-                      hipStream_t st) {
+// arrays and the cost table's choices; fl = first leaf rank of every cluster
+// (device).
+rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint32_t* fl, const uint32_t* pos,
+                      const uint32_t* bidx, hipStream_t st) {
     const uint32_t nc = s->n_clusters;
     const uint32_t root = nc - 1;
     const uint32_t cap = std::max<uint32_t>(s->n_trav, 1);  // wide nodes are a subset of the binary ones
-    uint32_t *items = nullptr, *next = nullptr, *children = nullptr, *nchild = nullptr, *nexp = nullptr,
-             *offs = nullptr, *total = nullptr;
+    if (nc > s->n_tris && !choice) return fail(ctx, RTQ_EINTERNAL, "wide collapse without a cost table");
+    uint32_t *items = nullptr, *next = nullptr;
     Temps tmp(ctx);
     RTQ_HIP(ctx, tmp.get(&items, cap));
     RTQ_HIP(ctx, tmp.get(&next, cap));
-    RTQ_HIP(ctx, tmp.get(&children, (size_t)cap * kWidth));
-    RTQ_HIP(ctx, tmp.get(&nchild, cap));
-    RTQ_HIP(ctx, tmp.get(&nexp, cap));
-    RTQ_HIP(ctx, tmp.get(&offs, cap));
-    RTQ_HIP(ctx, tmp.get(&total, 1));
+    // tile status words of every level's scan: a level of c nodes takes
+    // ceil(c / kThreads) of them, and there are at most depth + 1 levels
+    const uint32_t n_status = cap / kThreads + s->depth + 2;
+    uint64_t* status = nullptr;
+    RTQ_HIP(ctx, tmp.get(&status, n_status));
+    RTQ_HIP(ctx, hipMemsetAsync(status, 0, sizeof(uint64_t) * n_status, st));
     // a refit writes the same buffer again (its size depends on the topology only)
-    if (!s->wide) RTQ_HIP(ctx, dev_alloc(&s->wide, 8ull * cap));
-    if (!s->wide_trav) RTQ_HIP(ctx, dev_alloc(&s->wide_trav, cap));
+    if (!s->wide) RTQ_HIP(ctx, take(s->kept_wide, &s->wide, 8ull * cap));
+    if (!s->wide_trav) RTQ_HIP(ctx, take(s->kept_wide_trav, &s->wide_trav, cap));
     s->wide_ordered = false;   // the nodes are rewritten with empty tables
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, items, root);
     RTQ_HIP(ctx, hipGetLastError());
-    // cost table of the collapse, bottom-up over the PLOC passes
-    const uint32_t nt = s->n_tris, max_leaf = s->params.max_leaf_size;
-    uint32_t* choice = nullptr;
-    if (nc > nt) {
-        float* cost = nullptr;
-        RTQ_HIP(ctx, tmp.get(&cost, (size_t)(nc - nt) * kWidth));
-        RTQ_HIP(ctx, tmp.get(&choice, nc - nt));
-        const uint32_t n_passes = (uint32_t)s->passes.size();
-        for (uint32_t k = 0; k < s->small_from; ++k) {
-            const auto& pr = s->passes[k];
-            hipLaunchKernelGGL(k_dp_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second, nt,
-                               max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
-            RTQ_HIP(ctx, hipGetLastError());
-        }
-        if (s->small_from < n_passes) {
-            hipLaunchKernelGGL(k_dp_small, dim3(1), dim3(kDpSmallThreads), 0, st, s->d_passes, s->small_from, n_passes,
-                               nt, max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
-            RTQ_HIP(ctx, hipGetLastError());
-        }
-    }
-    // levels top-down, kLevelsPerRead at a time between host reads: a level's
-    // count and first index stay on the device (lvl), its launches cover an
-    // upper bound (8 children per node, at most cap nodes)
-    constexpr uint32_t kLevelsPerRead = 4;
+    // levels top-down, one launch each, kLevelsPerRead of them between host
+    // reads: a level's words (count, first node, first status word, ticket)
+    // stay on the device, its launch covers an upper bound of its count (8
+    // children per node, at most cap nodes)
+    constexpr uint32_t kLevelsPerRead = 6;
+    constexpr uint32_t kLvWords = 4 * (kLevelsPerRead + 1);
     uint32_t* lvl = nullptr;
-    RTQ_HIP(ctx, tmp.get(&lvl, 2 * (kLevelsPerRead + 1) + 1 + kLevelsPerRead));
-    uint32_t* err = lvl + 2 * (kLevelsPerRead + 1);
-    uint32_t* totals = err + 1;
-    uint32_t level_items = 1, level_base = 0, levels = 0;
+    RTQ_HIP(ctx, tmp.get(&lvl, kLvWords + 1));
+    uint32_t* err = lvl + kLvWords;
+    LevelArgs la{};
+    la.status = status;
+    la.n_status = n_status;
+    la.cap = cap;
+    la.err = err;
+    la.max_leaf = s->params.max_leaf_size;
+    la.nt = s->n_tris;
+    la.pad = s->box_pad;
+    la.cleft = s->cleft;
+    la.cright = s->cright;
+    la.ccount = s->ccount;
+    la.cbox = s->cbox;
+    la.choice = choice;
+    la.fl = fl;
+    la.pos = pos;
+    la.bidx = bidx;
+    la.nodes = s->wide;
+    la.wide_trav = s->wide_trav;
+    uint32_t level_items = 1, level_base = 0, level_status = 0, levels = 0, n_wide = 0;
     while (level_items > 0) {
-        const uint32_t h_head[3] = {level_items, level_base, 0u};
-        RTQ_HIP(ctx, hipMemcpyAsync(lvl, h_head, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        RTQ_HIP(ctx, hipMemcpyAsync(err, h_head + 2, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        // the group's first level; the others (and the error word) zero until written
+        uint32_t h_head[kLvWords + 1] = {};
+        h_head[0] = level_items;
+        h_head[1] = level_base;
+        h_head[2] = level_status;
+        RTQ_HIP(ctx, hipMemcpyAsync(lvl, h_head, sizeof h_head, hipMemcpyHostToDevice, st));
         uint64_t ub = level_items;
         for (uint32_t g = 0; g < kLevelsPerRead; ++g) {
-            const uint32_t u = (uint32_t)ub;
-            uint32_t* lv = lvl + 2 * g;
-            hipLaunchKernelGGL(k_wide_collapse, dim3(blocks_for(u)), dim3(kThreads), 0, st, u, lv, items,
-                               s->params.max_leaf_size, s->cleft, s->cright, s->ccount, s->cbox, nt, choice, children,
-                               nchild, nexp);
-            RTQ_HIP(ctx, hipGetLastError());
-            rtq_status rs = scan_u32(ctx, nexp, offs, u, totals + g, st);
-            if (rs != RTQ_OK) return rs;
-            hipLaunchKernelGGL(k_wide_write, dim3(blocks_for(u)), dim3(kThreads), 0, st, u, lv, cap, err,
-                               s->params.max_leaf_size, s->box_pad, items, children, nchild, offs, s->cleft,
-                               s->ccount, s->cbox, fl, pos, bidx, s->wide, s->wide_trav, next);
-            RTQ_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, st, lv, totals + g, cap, err, lv + 2);
+            la.lv = lvl + 4 * g;
+            la.items = items;
+            la.next_items = next;
+            hipLaunchKernelGGL(k_wide_level, dim3(blocks_for(ub)), dim3(kThreads), 0, st, la);
             RTQ_HIP(ctx, hipGetLastError());
             std::swap(items, next);
             ub = std::min<uint64_t>(8 * ub, cap);
         }
-        uint32_t h_lvl[2 * (kLevelsPerRead + 1) + 1];
-        RTQ_HIP(ctx, hipMemcpyAsync(h_lvl, lvl, sizeof h_lvl, hipMemcpyDeviceToHost, st));
-        RTQ_HIP(ctx, hipStreamSynchronize(st));
-        if (h_lvl[2 * (kLevelsPerRead + 1)]) return fail(ctx, RTQ_EINTERNAL, "wide collapse overflow");
+        uint32_t h_lvl[kLvWords + 1];
+        HostReads rd(ctx, st);
+        RTQ_HIP(ctx, rd.add(h_lvl, lvl, sizeof h_lvl));
+        RTQ_HIP(ctx, rd.wait());
+        if (h_lvl[kLvWords] & kWideErrScan) return fail(ctx, RTQ_EINTERNAL, "wide collapse: a level scan gave up");
+        if (h_lvl[kLvWords]) return fail(ctx, RTQ_EINTERNAL, "wide collapse overflow");
         for (uint32_t g = 0; g < kLevelsPerRead; ++g)
-            if (h_lvl[2 * g] > 0) ++levels;
-        level_items = h_lvl[2 * kLevelsPerRead];
-        level_base = h_lvl[2 * kLevelsPerRead + 1];
+            if (h_lvl[4 * g] > 0) {
+                ++levels;
+                n_wide = h_lvl[4 * g + 1] + h_lvl[4 * g];
+            }
+        level_items = h_lvl[4 * kLevelsPerRead];
+        level_base = h_lvl[4 * kLevelsPerRead + 1];
+        level_status = h_lvl[4 * kLevelsPerRead + 2];
     }
-    s->n_wide = level_base;
+    s->n_wide = n_wide;
     s->wide_depth = levels;
     return RTQ_OK;
 }

@@ -69,9 +69,10 @@ WIDE_NODE_BYTES = 96     # an 8-wide visit reads 96 B of its 128-B slot
 LEARNED_WIDE_NODE_BYTES = 104   # with the learned first-child table (bytes 96-103)
 NODE_BYTES = {"binary": BINARY_NODE_BYTES, "wide": WIDE_NODE_BYTES, "learned": BINARY_NODE_BYTES,
               "learned_wide": LEARNED_WIDE_NODE_BYTES}
-# sources the traversal kernels' memory traffic depends on; the committed
-# traffic summary records their digest and is ignored when it differs
-KERNEL_SOURCES = ("trace.hip", "wide.hip", "build.hip", "batch.hip", "rtq_internal.hpp")
+# sources of the traversal kernels; the committed traffic summary records
+# their digest and the fingerprint of the terrain tree it was measured on
+# (tree_digest), and is ignored when either differs
+KERNEL_SOURCES = ("trace.hip", "batch.hip", "rtq_internal.hpp")
 
 
 def kernel_label(rocprof_name: str):
@@ -87,7 +88,7 @@ def kernel_label(rocprof_name: str):
     return f"k_trace<{leaf},{mode},{layout}" + (",batch>" if m.group(5) != "0" else ">")
 
 
-def summarize_profile(out_path: str, dirs: list[str], commands: list[str], last: int = 5) -> str:
+def summarize_profile(out_path: str, dirs: list[str], commands: list[str], last: int = 5, tree=None) -> str:
     """Writes the traffic summary read by read_profile from rocprofv3 --pmc
     CSVs (FETCH_SIZE, WRITE_SIZE, TCC_READ_SECTORS_sum + TCC_WRITE_SECTORS_sum
     passes of the same runs, each ending with a 512 MiB device copy): per
@@ -138,6 +139,8 @@ def summarize_profile(out_path: str, dirs: list[str], commands: list[str], last:
              "# commands (repository root $R, from /tmp with TMPDIR=/tmp):"]
     lines += [f"#   {c}" for c in commands]
     lines.append(f"kernel_sources_sha256: {kernel_digest()}")
+    if tree:
+        lines.append(f"tree_sha256: {tree}")
     labels = sorted({lab for c in vals for lab in vals[c]})
     def get(counter, lab):
         v = vals[counter].get(lab)
@@ -182,6 +185,8 @@ def parse():
     ap.add_argument("--summarize-profile", nargs="+", metavar=("OUT", "PMC_DIR"),
                     help="write the traffic summary OUT from rocprofv3 --pmc output directories and exit")
     ap.add_argument("--profile-commands", nargs="*", default=[], help="commands recorded in the summary")
+    ap.add_argument("--profile-tree-log", default=None,
+                    help="output of one of the profiled bench runs: the summary records its terrain tree")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device-to-device 3 times: a kernel of "
                          "known traffic to scale rocprof FETCH_SIZE / WRITE_SIZE readings")
@@ -197,13 +202,39 @@ def kernel_digest() -> str:
     return h.hexdigest()[:16]
 
 
+def tree_digest(sc) -> str:
+    """SHA-256 (first 16 hex digits) of a built scene's traversal layouts: the
+    binary nodes with their leaf ranges, and the 8-wide nodes if built."""
+    import hashlib
+    h = hashlib.sha256()
+    nodes, ranges = sc.export_trav()
+    h.update(nodes.tobytes())
+    h.update(ranges.tobytes())
+    if sc.info().n_wide_nodes:
+        h.update(sc.export_wide().tobytes())
+    return h.hexdigest()[:16]
+
+
+def tree_of_log(path: str):
+    """The tree fingerprint a bench run printed (detail.tree_sha256 of its JSON
+    line), or None."""
+    for line in Path(path).read_text(errors="replace").splitlines():
+        if line.startswith("{") and "tree_sha256" in line:
+            try:
+                return json.loads(line)["detail"]["tree_sha256"]
+            except (ValueError, KeyError, TypeError):
+                continue
+    return None
+
+
 def read_profile(path: str) -> dict:
     """Per-kernel bytes per launch from the committed rocprof counter summary:
     `hbm_bytes_per_launch <kernel>: v` (calibrated FETCH_SIZE + WRITE_SIZE) and
     `l2_bytes_per_launch <kernel>: v` (TCC read + write sectors x 32 B), with
-    the kernel-source digest they were measured at (`kernel_sources_sha256: d`).
-    `fresh` is False when the digest differs from the current sources."""
-    out = {"hbm": {}, "l2": {}, "digest": None, "fresh": False, "path": path}
+    the kernel-source digest and terrain tree they were measured at
+    (`kernel_sources_sha256: d`, `tree_sha256: t`).  `fresh` is False when the
+    digest differs from the current sources; the caller compares the tree."""
+    out = {"hbm": {}, "l2": {}, "digest": None, "tree": None, "fresh": False, "path": path, "stale": None}
     p = Path(path)
     if not p.exists():
         return out
@@ -214,8 +245,20 @@ def read_profile(path: str) -> dict:
                 out[field][k.strip()] = float(v)
         if line.startswith("kernel_sources_sha256:"):
             out["digest"] = line.split(":", 1)[1].strip()
+        if line.startswith("tree_sha256:"):
+            out["tree"] = line.split(":", 1)[1].strip()
     out["fresh"] = out["digest"] == kernel_digest()
+    if not out["fresh"]:
+        out["stale"] = (f"{p.name} was measured at kernel sources {out['digest']}, current {kernel_digest()}")
     return out
+
+
+def check_tree(prof: dict, tree: str) -> dict:
+    """Marks the summary stale when it was measured on another terrain tree."""
+    if prof["fresh"] and prof["tree"] != tree:
+        prof["fresh"] = False
+        prof["stale"] = f"{Path(prof['path']).name} was measured on terrain tree {prof['tree']}, current {tree}"
+    return prof
 
 
 def roofline_of(prof: dict, kernel: str, launch_ms: float, alg_bytes: int, node_bytes: int) -> dict:
@@ -229,9 +272,8 @@ def roofline_of(prof: dict, kernel: str, launch_ms: float, alg_bytes: int, node_
     achieved = hbm / s / 1e9 if hbm else None
     l2_gbs = l2 / s / 1e9 if l2 else None
     src = (f"{Path(prof['path']).name}: rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated) and TCC sectors, kernel "
-           f"sources {prof['digest']}") if prof["fresh"] else \
-        (f"stale: {Path(prof['path']).name} was measured at kernel sources {prof['digest']}, "
-         f"current {kernel_digest()}")
+           f"sources {prof['digest']}, terrain tree {prof['tree']}") if prof["fresh"] else \
+        f"stale: {prof['stale'] or Path(prof['path']).name}"
     return {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": hbm, "kernel": kernel, "launch_ms": round(launch_ms, 4), "traffic_source": src,
@@ -252,6 +294,7 @@ class Workload:
         self.ctx, self.sp = ctx, stream
         self.sc = ctx.upload(tris, models)
         self.info = self.sc.build(16, max_leaf, 0, node_width=node_width)
+        self.tree = tree_digest(self.sc)
         n = cam.width * cam.height
         self.prim, self.phit = devmem.empty(n, RAY_DTYPE), devmem.empty(n, HIT_DTYPE)
         ctx.raygen_pinhole(cam, self.prim.data_ptr(), stream)
@@ -369,7 +412,9 @@ def t_error_stats(tris, rays, hits, n_sample=20_000, seed=0):
 def main() -> int:
     args = parse()
     if args.summarize_profile:
-        print(summarize_profile(args.summarize_profile[0], args.summarize_profile[1:], args.profile_commands))
+        tree = tree_of_log(args.profile_tree_log) if args.profile_tree_log else None
+        print(summarize_profile(args.summarize_profile[0], args.summarize_profile[1:], args.profile_commands,
+                                tree=tree))
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU only; the trace path "
@@ -434,7 +479,7 @@ def main() -> int:
     default_params = ctx.launch_params()
     counters_by_layout = {"binary": wl.counters(RTQ_TRACE_BINARY, BINARY_NODE_BYTES),
                           "wide": wl.counters(0, WIDE_NODE_BYTES)}
-    profile = read_profile(args.traffic_file)
+    profile = check_tree(read_profile(args.traffic_file), wl.tree)
     reference = None
     results = {}
     learned_setup = None
@@ -907,7 +952,7 @@ def main() -> int:
                    "cpu_reference_case": cfg0,
                    "build_ms": round(info.build_ms, 2), "ploc_passes": info.ploc_iterations,
                    "depth": info.depth, "trav_nodes": info.n_trav_nodes, "wide_nodes": info.n_wide_nodes,
-                   "wide_depth": info.wide_depth, "setup_s": round(setup_s, 1)},
+                   "wide_depth": info.wide_depth, "tree_sha256": wl.tree, "setup_s": round(setup_s, 1)},
     }
     print(json.dumps(out))
     ctx.close()

@@ -25,10 +25,14 @@ def test_more_than_one_gpu_exits_with_a_message():
 
 def test_read_profile_and_roofline(tmp_path):
     f = tmp_path / "t.txt"
-    f.write_text(f"# comment\nkernel_sources_sha256: {bench.kernel_digest()}\n"
+    f.write_text(f"# comment\nkernel_sources_sha256: {bench.kernel_digest()}\ntree_sha256: 0123abcd\n"
                  "hbm_bytes_per_launch k_a: 2000000000\nl2_bytes_per_launch k_a: 9.8304e9\n")
     p = bench.read_profile(str(f))
-    assert p["fresh"] and p["hbm"]["k_a"] == 2e9 and p["l2"]["k_a"] == 9.8304e9
+    assert p["fresh"] and p["hbm"]["k_a"] == 2e9 and p["l2"]["k_a"] == 9.8304e9 and p["tree"] == "0123abcd"
+    # measured on another tree: stale, with the reason
+    other = bench.check_tree(bench.read_profile(str(f)), "ffff0000")
+    assert not other["fresh"] and "ffff0000" in bench.roofline_of(other, "k_a", 1.0, 1, 96)["traffic_source"]
+    assert bench.check_tree(p, "0123abcd")["fresh"]
     r = bench.roofline_of(p, "k_a", 1.0, 17_000_000_000, 96)
     assert r["achieved"] == 2000.0 and r["frac"] == 0.25 and r["traffic"] == 2e9
     assert r["algorithmic"]["GB_s"] == 17000.0 and r["l2"]["GB_s"] == 9830.4
@@ -55,12 +59,22 @@ def test_kernel_labels():
     assert bench.kernel_label("__amd_rocclr_copyBuffer") is None
 
 
+def test_tree_of_log(tmp_path):
+    f = tmp_path / "run.log"
+    f.write_text("rocprofv3 noise\n" + json.dumps({"metric": "m", "detail": {"tree_sha256": "abc123"}}) +
+                 "\n" + json.dumps({"copy_calibration_bytes": 1}) + "\n")
+    assert bench.tree_of_log(str(f)) == "abc123"
+    f.write_text("no bench line\n")
+    assert bench.tree_of_log(str(f)) is None
+
+
 def test_committed_traffic_covers_every_reported_kernel():
-    """The committed summary was measured on the current kernel sources and
-    holds HBM and L2 bytes for every kernel the bench reports; its HBM rate
-    for the headline launch stays below the HBM peak."""
+    """The committed summary was measured on the current kernel sources (the
+    bench checks the terrain tree it records at run time) and holds HBM and L2
+    bytes for every kernel the bench reports."""
     prof = bench.read_profile(str(ROOT / "profiles" / "traffic_terrain_trace.txt"))
     assert prof["fresh"], f"re-profile: summary digest {prof['digest']} != sources {bench.kernel_digest()}"
+    assert prof["tree"], "the summary records the terrain tree it was measured on"
     names = [f"k_trace<watertight,{m},{l}>" for m in ("any", "closest") for l in ("binary", "wide")]
     names += [f"k_trace<watertight,per-item,{l},batch>" for l in ("binary", "wide")]
     for n in names:

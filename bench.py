@@ -261,6 +261,52 @@ def check_tree(prof: dict, tree: str) -> dict:
     return prof
 
 
+# per-lane vector-memory operations of one visit, by node layout: 16-B loads
+# of an 8-wide node (96 B; + one 8-B load of the learned table), 16-B loads of
+# a 64-B binary node (visits of the top-of-tree table read LDS: an upper bound)
+VMEM_OPS_PER_VISIT = {"wide": 6, "learned_wide": 7, "binary": 4, "learned": 4}
+VMEM_OPS_PER_TRI = 3        # three 16-B vertex loads
+VMEM_OPS_PER_RAY = 3        # two 16-B ray loads, one 16-B hit store
+
+
+def probe_vmem(slots_miss: int = 219_000, slots_hit: int = 64) -> dict | None:
+    """Per-lane 16-B load rates of dependent chains (librtqprobe.so, six loads
+    per 128-B slot as an 8-wide visit, 6 workgroups of 256 lanes per CU):
+    over 28 MB of slots (loads miss L1) and over 64 slots (loads hit L1)."""
+    import ctypes
+    lib_path = ROOT / "realtimeraytracing_amd" / "librtqprobe.so"
+    if not lib_path.exists():
+        return None
+    lib = ctypes.CDLL(str(lib_path))
+    fn = lib.rtqprobe_vmem_rate
+    fn.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    fn.restype = ctypes.c_int
+    out = {}
+    for name, slots in (("l1_miss", slots_miss), ("l1_hit", slots_hit)):
+        r = ctypes.c_double(0.0)
+        rc = fn(0, 6, slots, 6, ctypes.byref(r))
+        if rc != 0:
+            raise RuntimeError(f"vector-memory probe failed: HIP error {rc}")
+        out[name] = r.value
+    return out
+
+
+def vmem_of(sets: list, layout: str, launch_ms: float, probe) -> dict:
+    """Per-lane vector-memory operations of one launch (from the traversal
+    counters of its ray sets) per second, against the probe's rates."""
+    ops = sum(VMEM_OPS_PER_VISIT[layout] * c["node_visits"] + VMEM_OPS_PER_TRI * c["tri_tests"] +
+              VMEM_OPS_PER_RAY * c["rays"] for c in sets)
+    rate = ops / (launch_ms * 1e-3)
+    out = {"lane_ops_per_launch": int(ops), "achieved_G_per_s": round(rate / 1e9, 1),
+           "ops": f"{VMEM_OPS_PER_VISIT[layout]} x node visits + {VMEM_OPS_PER_TRI} x triangle tests + "
+                  f"{VMEM_OPS_PER_RAY} x rays (16-B lane loads / stores)"}
+    if probe:
+        out.update({"probe_G_per_s": {k: round(v / 1e9, 1) for k, v in probe.items()},
+                    "frac_of_l1_miss_rate": round(rate / probe["l1_miss"], 3),
+                    "frac_of_l1_hit_rate": round(rate / probe["l1_hit"], 3)})
+    return out
+
+
 def roofline_of(prof: dict, kernel: str, launch_ms: float, alg_bytes: int, node_bytes: int) -> dict:
     """HBM roofline of one launch: calibrated DRAM bytes over its HIP-event
     time against the 8 TB/s peak; beside it the algorithmic bytes (ray + hit +
@@ -480,6 +526,7 @@ def main() -> int:
     counters_by_layout = {"binary": wl.counters(RTQ_TRACE_BINARY, BINARY_NODE_BYTES),
                           "wide": wl.counters(0, WIDE_NODE_BYTES)}
     profile = check_tree(read_profile(args.traffic_file), wl.tree)
+    vmem_probe = probe_vmem()
     reference = None
     results = {}
     learned_setup = None
@@ -498,6 +545,7 @@ def main() -> int:
             dom = "shadow" if shad_ms >= diff_ms else "diffuse"
             dom_ms = shad_ms if dom == "shadow" else diff_ms
             dom_bytes = counters[dom]["bytes"]
+            dom_sets = [counters[dom]]
             kernel = f"k_trace<watertight,{'any' if dom == 'shadow' else 'closest'},{lname}>"
         elif form in ("batched", "learned"):
             # both ray sets in one persistent launch, replayed from a hipGraph,
@@ -535,6 +583,7 @@ def main() -> int:
             dom = "batch"
             dom_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
             dom_bytes = counters["diffuse"]["bytes"] + counters["shadow"]["bytes"]
+            dom_sets = [counters["diffuse"], counters["shadow"]]
             kernel = f"k_trace<watertight,per-item,{lname},batch>"
         else:
             raise SystemExit(f"unknown form {form}")
@@ -548,6 +597,7 @@ def main() -> int:
             "layout": lname, "tuned": tuned, "counters": counters, "exact": same_results(reference, got),
             "roofline": roofline_of(profile, kernel, dom_ms, dom_bytes, node_bytes),
         }
+        results[form]["roofline"]["vmem"] = vmem_of(dom_sets, lname, dom_ms, vmem_probe)
     ctx.set_launch_params(default_params.refill, default_params.chunk, bool(default_params.prefer_wide),
                               default_params.spec_slack, default_params.leaf_slack,
                               default_params.stack_lds)

@@ -81,6 +81,24 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+PROBE_SRC = CSRC / "probe" / "vmem_rate.hip"
+PROBE_LIB = PKG / "librtqprobe.so"
+
+
+def build_probe(verbose: bool = False) -> Path:
+    """The vector-memory rate probe bench.py measures the trace kernels'
+    ceiling with (its own small library; librtq does not link it)."""
+    tmp = PROBE_LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), *compile_flags(), "-shared", "-o", str(tmp), str(PROBE_SRC)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"probe build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, PROBE_LIB)
+    if verbose:
+        print(f"built {PROBE_LIB}")
+    return PROBE_LIB
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """The CPU checker (test infrastructure, never loaded by the library)."""
     res = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
@@ -113,6 +131,7 @@ def build_cpp_check(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_probe(verbose=True)
     if "--with-oracle" in sys.argv:
         build_oracle(verbose=True)
         build_cpp_check(verbose=True)

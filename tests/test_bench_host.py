@@ -47,6 +47,17 @@ def test_read_profile_and_roofline(tmp_path):
     assert not bench.read_profile(str(tmp_path / "missing.txt"))["fresh"]
 
 
+def test_vmem_ops():
+    sets = [{"rays": 10, "node_visits": 100, "tri_tests": 20}, {"rays": 5, "node_visits": 40, "tri_tests": 0}]
+    v = bench.vmem_of(sets, "wide", 1.0, {"l1_miss": 1e6, "l1_hit": 2e6})
+    ops = 6 * 140 + 3 * 20 + 3 * 15
+    assert v["lane_ops_per_launch"] == ops
+    assert v["frac_of_l1_miss_rate"] == round(ops * 1e3 / 1e6, 3)
+    assert v["frac_of_l1_hit_rate"] == round(ops * 1e3 / 2e6, 3)
+    assert bench.vmem_of(sets, "learned_wide", 1.0, None)["lane_ops_per_launch"] == ops + 140
+    assert "probe_G_per_s" not in bench.vmem_of(sets, "binary", 1.0, None)
+
+
 def test_kernel_labels():
     assert bench.kernel_label("void rtq::(anonymous namespace)::k_trace<0, 1, false, 0, 0>(rtq::x)") == \
         "k_trace<watertight,any,binary>"

@@ -136,13 +136,12 @@ rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params
  * traversal node indices), every box recomputed bottom-up for the scene's
  * current vertices (leaf = triangle box, bvh.cpp:383-399; parent = merge of its
  * children, bvh.cpp:401-413), then the reference node array, the traversal
- * nodes and triangles and the compressed wide nodes rewritten in their
- * buffers, so batch graphs captured before stay valid.  The 8-wide collapse is
- * recomputed from the new boxes, so the 8-wide node count and depth may change
- * (never beyond the binary traversal node count and depth, which bound its
- * buffer and stacks).  Learned order words are cleared.  Hit records stay exact
- * (they never depend on the tree); traversal cost grows as the vertices move
- * away from the ones the tree was built for. */
+ * nodes and triangles rewritten in their buffers, so batch graphs captured
+ * before stay valid.  The 8-wide nodes keep their children (node count, depth,
+ * child refs) and get boxes quantised from the new cluster boxes.  Learned
+ * order words and tables are cleared.  Hit records stay exact (they never
+ * depend on the tree); traversal cost grows as the vertices move away from
+ * the ones the tree was built for. */
 rtq_status rtq_bvh_refit(rtq_ctx* ctx, uint32_t scene_id);
 
 typedef struct rtq_bvh_info {

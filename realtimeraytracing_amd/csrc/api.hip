@@ -57,6 +57,7 @@ void reset_build(Scene* s) {
     s->node_aux = nullptr;
     s->wide = nullptr;
     s->wide_trav = nullptr;
+    s->wide_kids = nullptr;
     s->wide_ordered = false;
     s->d_passes = nullptr;
     s->small_from = 0;
@@ -86,6 +87,7 @@ void free_build(Scene* s) {
     release(s->kept_aux);
     release(s->kept_wide);
     release(s->kept_wide_trav);
+    release(s->kept_wide_kids);
     release(s->kept_passes);
     release(s->cbox);
     release(s->ctri);
@@ -97,6 +99,8 @@ void free_build(Scene* s) {
     release(s->sorted_codes);
     release(s->ref_nodes);
     release(s->preorder);
+    release(s->first_leaf);
+    release(s->node_depth);
     release(s->top);
     release(s->tris);
 }

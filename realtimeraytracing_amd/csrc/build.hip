@@ -1269,21 +1269,26 @@ private:
     bool forked_ = false, joined_ = false;
 };
 
-// flag_dev (optional): one more device word to read back with the layout's
-// own host read, into *flag_host.
-rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipStream_t st,
+// refit: the tree is the one the last build wrote the layouts for, with new
+// boxes: its pre-order placement is kept and the 8-wide nodes keep their
+// children.  flag_dev (optional): one more device word to read back with the
+// layout's own host read, into *flag_host.
+rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipStream_t st, bool refit,
                          const uint32_t* flag_dev = nullptr, uint32_t* flag_host = nullptr) {
     const uint32_t n = s->n_tris;
     const uint32_t nc = s->n_clusters;
     Temps tmp(ctx);
-    uint32_t *fl = nullptr, *dep = nullptr, *maxd = nullptr, *flag = nullptr, *bidx = nullptr, *ntrav = nullptr;
-    // pre-order positions stay with the scene: the reference node array is
-    // written from them on demand (ensure_ref_nodes)
+    uint32_t *maxd = nullptr, *flag = nullptr, *bidx = nullptr, *ntrav = nullptr;
+    // pre-order positions, first leaf ranks and depths stay with the scene:
+    // the reference node array is written from them on demand
+    // (ensure_ref_nodes), and a refit keeps them
     if (!s->preorder) RTQ_HIP(ctx, dev_alloc(&s->preorder, nc));
-    uint32_t* pos = s->preorder;
+    if (!s->first_leaf) RTQ_HIP(ctx, dev_alloc(&s->first_leaf, nc));
+    if (!s->node_depth) RTQ_HIP(ctx, dev_alloc(&s->node_depth, nc));
+    uint32_t* const pos = s->preorder;
+    uint32_t* const fl = s->first_leaf;
+    uint32_t* const dep = s->node_depth;
     s->ref_ready = false;
-    RTQ_HIP(ctx, tmp.get(&fl, nc));
-    RTQ_HIP(ctx, tmp.get(&dep, nc));
     RTQ_HIP(ctx, tmp.get(&maxd, 2));
     uint32_t* bdep = nullptr;
     RTQ_HIP(ctx, tmp.get(&bdep, blocks_for(nc)));
@@ -1297,7 +1302,7 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     float* wcost = nullptr;
     uint32_t* wchoice = nullptr;
     SideWork side(ctx);
-    const uint32_t dp_steps = s->params.node_width == 8 ? wide_cost_steps(s) : 0u;
+    const uint32_t dp_steps = s->params.node_width == 8 && !refit ? wide_cost_steps(s) : 0u;
     uint32_t dp_next = 0;
     auto queue_cost = [&](uint32_t upto) -> rtq_status {
         upto = std::min(upto, dp_steps);
@@ -1311,25 +1316,28 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
         RTQ_HIP(ctx, tmp.get(&wchoice, nc - n));
         RTQ_HIP(ctx, side.fork(st));
     }
-    hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
-    RTQ_HIP(ctx, hipGetLastError());
-    // the small passes at the top of the tree in one launch, then one launch
-    // per larger pass, last pass first
-    const uint32_t n_passes = (uint32_t)s->passes.size();
-    if (s->small_from < n_passes) {
-        hipLaunchKernelGGL(k_place_small, dim3(1), dim3(kSmallThreads), 0, st, s->d_passes, s->small_from, n_passes,
-                           s->cleft, s->cright, s->ccount, pos, fl, dep);
+    rtq_status rs = RTQ_OK;
+    if (!refit) {
+        hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
         RTQ_HIP(ctx, hipGetLastError());
-    }
-    rtq_status rs = queue_cost(2);
-    if (rs != RTQ_OK) return rs;
-    for (uint32_t k = s->small_from; k > 0; --k) {
-        const auto& pr = s->passes[k - 1];
-        hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
-                           s->cleft, s->cright, s->ccount, pos, fl, dep);
-        RTQ_HIP(ctx, hipGetLastError());
-        rs = queue_cost(dp_next + 1);
+        // the small passes at the top of the tree in one launch, then one
+        // launch per larger pass, last pass first
+        const uint32_t n_passes = (uint32_t)s->passes.size();
+        if (s->small_from < n_passes) {
+            hipLaunchKernelGGL(k_place_small, dim3(1), dim3(kSmallThreads), 0, st, s->d_passes, s->small_from,
+                               n_passes, s->cleft, s->cright, s->ccount, pos, fl, dep);
+            RTQ_HIP(ctx, hipGetLastError());
+        }
+        rs = queue_cost(2);
         if (rs != RTQ_OK) return rs;
+        for (uint32_t k = s->small_from; k > 0; --k) {
+            const auto& pr = s->passes[k - 1];
+            hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first,
+                               pr.second, s->cleft, s->cright, s->ccount, pos, fl, dep);
+            RTQ_HIP(ctx, hipGetLastError());
+            rs = queue_cost(dp_next + 1);
+            if (rs != RTQ_OK) return rs;
+        }
     }
     hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->ctri, s->cleft, fl,
                        dep, s->tri_world, s->tris, bdep);
@@ -1377,8 +1385,12 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
         RTQ_HIP(ctx, hipGetLastError());
     }
     if (s->params.node_width == 8) {
-        RTQ_HIP(ctx, side.join(st));
-        rs = build_wide(ctx, s, wchoice, fl, pos, n == 1 ? nullptr : bidx, st);
+        if (refit) {
+            rs = refit_wide(ctx, s, st);
+        } else {
+            RTQ_HIP(ctx, side.join(st));
+            rs = build_wide(ctx, s, wchoice, fl, pos, n == 1 ? nullptr : bidx, st);
+        }
         if (rs != RTQ_OK) return rs;
     }
     rs = write_top(ctx, s, st);
@@ -1414,14 +1426,14 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     s->n_clusters = nc;
     // size the context's scratch for this build's temporaries up front, so a
     // first build does not allocate them one by one.  Alive together, per
-    // triangle: sort keys and values and the PLOC lists 64 B, placement 32 B;
-    // with 8-wide nodes also the collapse cost table 36 B and the collapse's
-    // two level lists 8 B per binary traversal node (one per triangle with
-    // leaves of one triangle); the fused PLOC passes' tile status words (16
-    // passes x 8 B per tile of 480), each placement workgroup's deepest leaf
-    // and the collapse's tile status words, under n / 2 B; 1 MiB for scans and
-    // 256-B rounding
-    const size_t per_tri = prm.node_width == 8 ? 64 + 32 + 36 + 8 : 64 + 32;
+    // triangle: sort keys and values and the PLOC lists 64 B, the traversal
+    // node marks and indices 16 B; with 8-wide nodes also the collapse cost
+    // table 36 B and the collapse's two level lists 8 B per binary traversal
+    // node (one per triangle with leaves of one triangle); the fused PLOC
+    // passes' tile status words (16 passes x 8 B per tile of 480), each
+    // placement workgroup's deepest leaf and the collapse's tile status
+    // words, under n / 2 B; 1 MiB for scans and 256-B rounding
+    const size_t per_tri = prm.node_width == 8 ? 64 + 16 + 36 + 8 : 64 + 16;
     ctx->arena.peak = std::max<size_t>(ctx->arena.peak, per_tri * n + n / 2 + ((size_t)1 << 20));
     rtq_status rs = arena_reserve(ctx);
     if (rs != RTQ_OK) return rs;
@@ -1500,7 +1512,7 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
 
     // 3-4. node arrays of every layout
     uint32_t h_sort_err = 0;
-    rs = write_layouts(ctx, s, d_bounds, st, sort_err, &h_sort_err);
+    rs = write_layouts(ctx, s, d_bounds, st, false, sort_err, &h_sort_err);
     if (rs != RTQ_OK) return rs;
     if (h_sort_err) return fail(ctx, RTQ_EINTERNAL, "build: a radix sort pass's look-back gave up");
     RTQ_HIP(ctx, hipEventRecord(ev1, st));
@@ -1535,8 +1547,9 @@ rtq_status write_top(rtq_ctx* ctx, Scene* s, hipStream_t st) {
 
 // New boxes for the kept cluster tree (rtq_bvh_refit): leaf boxes from the
 // current vertices, then the merges of each PLOC pass in build order (a
-// merge's children come from earlier passes), then every node array again,
-// written into the buffers it already has.
+// merge's children come from earlier passes), then the leaf triangles and the
+// binary nodes again and new boxes in the kept 8-wide nodes, written into the
+// buffers they already have.
 rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
     hipStream_t st = ctx->stream;
     const uint32_t n = s->n_tris;
@@ -1562,7 +1575,7 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
     }
     rtq_status rs = compute_bounds(ctx, s, s->params.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
     if (rs != RTQ_OK) return rs;
-    return write_layouts(ctx, s, d_bounds, st);
+    return write_layouts(ctx, s, d_bounds, st, true);
 }
 
 rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_codes) {

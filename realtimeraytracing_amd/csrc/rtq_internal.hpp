@@ -97,6 +97,8 @@ struct Scene {
     rtq_bvh_node* ref_nodes = nullptr;  // 2N - 1 pre-order nodes (reference layout), written on demand
     bool ref_ready = false;             // ref_nodes matches the current tree and boxes
     uint32_t* preorder = nullptr;       // pre-order position of every cluster (2N - 1)
+    uint32_t* first_leaf = nullptr;     // rank of the first leaf below every cluster (2N - 1)
+    uint32_t* node_depth = nullptr;     // depth of every cluster (2N - 1)
     float4* nodes = nullptr;            // traversal nodes, 4 float4 (64 B) each
     uint4* node_aux = nullptr;          // per traversal node: child leaf ranges [A,B) [B,C), depth
     bool ordered = false;               // learned child-order words stored in the nodes
@@ -108,6 +110,7 @@ struct Scene {
     uint32_t n_trav = 0;
     float4* wide = nullptr;             // compressed 8-wide nodes, 8 float4 (128 B) each
     uint32_t* wide_trav = nullptr;      // binary traversal node of every 8-wide node's root cluster
+    uint32_t* wide_kids = nullptr;      // 8 child clusters of every 8-wide node (kNone past the last)
     bool wide_ordered = false;          // learned first-child tables written in the 8-wide nodes
     uint32_t n_wide = 0;
     uint32_t wide_depth = 0;            // levels of the wide tree
@@ -120,9 +123,9 @@ struct Scene {
     uint2* d_passes = nullptr;
     std::vector<uint2> h_passes;
     uint32_t small_from = 0;
-    // the memory behind nodes, node_aux, wide, wide_trav and d_passes, kept
-    // from build to build
-    Kept kept_nodes, kept_aux, kept_wide, kept_wide_trav, kept_passes;
+    // the memory behind nodes, node_aux, wide, wide_trav, wide_kids and
+    // d_passes, kept from build to build
+    Kept kept_nodes, kept_aux, kept_wide, kept_wide_trav, kept_wide_kids, kept_passes;
     // vertices changed since the last build or refit (rtq_scene_update)
     bool stale = false;
     float box_pad = 0.0f;
@@ -378,6 +381,9 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint
 uint32_t wide_cost_steps(const Scene* s);
 rtq_status wide_cost_table(rtq_ctx* ctx, const Scene* s, float* cost, uint32_t* choice, uint32_t first,
                            uint32_t last, hipStream_t st);
+// New boxes for the kept 8-wide nodes after a refit (the cluster boxes and
+// box pad already refitted): one launch over the nodes, child refs kept.
+rtq_status refit_wide(rtq_ctx* ctx, Scene* s, hipStream_t st);
 // Learned first-child tables of the 8-wide nodes from the binary nodes' order
 // words (learned.hip); clear = zero them.
 rtq_status write_wide_order(rtq_ctx* ctx, Scene* s, bool clear, hipStream_t st);

@@ -170,6 +170,55 @@ __device__ int exponent_for(double extent) {
     return e;
 }
 
+// The first four float4 of an 8-wide node with children ch[0..n) in slots
+// 0..n-1: origin and meta (exponents, valid mask), then the x, y and z planes.
+// Child boxes are widened by pad and quantised outward in exact (double)
+// arithmetic, so the decoded box contains the padded child box.
+__device__ void write_wide_boxes(const uint32_t* ch, uint32_t n, const rtq_aabb* __restrict__ cbox, float pad,
+                                 float4* __restrict__ o) {
+    float lo[kWidth][3], hi[kWidth][3];
+    float org[3] = {INFINITY, INFINITY, INFINITY};
+    double top[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; ++i) {
+        const rtq_aabb b = cbox[ch[i]];
+        for (int q = 0; q < 3; ++q) {
+            lo[i][q] = b.min[q] - pad;
+            hi[i][q] = b.max[q] + pad;
+            org[q] = fminf(org[q], lo[i][q]);
+            top[q] = fmax(top[q], (double)hi[i][q]);
+        }
+    }
+    int ex[3];
+    double scale[3];
+    for (int q = 0; q < 3; ++q) {
+        ex[q] = exponent_for(top[q] - (double)org[q]);
+        scale[q] = ldexp(1.0, ex[q]);
+    }
+    uint32_t qlo[3][2] = {{0, 0}, {0, 0}, {0, 0}}, qhi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    uint32_t valid = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        valid |= 1u << i;
+        for (int q = 0; q < 3; ++q) {
+            // exact: float differences and division by a power of two are exact in double
+            double l = floor(((double)lo[i][q] - (double)org[q]) / scale[q]);
+            double h = ceil(((double)hi[i][q] - (double)org[q]) / scale[q]);
+            l = fmin(fmax(l, 0.0), 255.0);
+            h = fmin(fmax(h, 0.0), 255.0);
+            qlo[q][i >> 2] |= (uint32_t)l << (8 * (i & 3));
+            qhi[q][i >> 2] |= (uint32_t)h << (8 * (i & 3));
+        }
+    }
+    const uint32_t meta = (uint32_t)(ex[0] + 127) | ((uint32_t)(ex[1] + 127) << 8) |
+                          ((uint32_t)(ex[2] + 127) << 16) | (valid << 24);
+    o[0] = make_float4(org[0], org[1], org[2], __uint_as_float(meta));
+    o[1] = make_float4(__uint_as_float(qlo[0][0]), __uint_as_float(qlo[0][1]), __uint_as_float(qhi[0][0]),
+                       __uint_as_float(qhi[0][1]));
+    o[2] = make_float4(__uint_as_float(qlo[1][0]), __uint_as_float(qlo[1][1]), __uint_as_float(qhi[1][0]),
+                       __uint_as_float(qhi[1][1]));
+    o[3] = make_float4(__uint_as_float(qlo[2][0]), __uint_as_float(qlo[2][1]), __uint_as_float(qhi[2][0]),
+                       __uint_as_float(qhi[2][1]));
+}
+
 // Tile status of the level scan: value in the low 32 bits, state on top.
 constexpr uint64_t kLvAgg = 1ull << 62;     // the tile's own count
 constexpr uint64_t kLvPre = 2ull << 62;     // the counts through the tile
@@ -199,6 +248,7 @@ struct LevelArgs {
     const uint32_t* bidx;
     float4* nodes;
     uint32_t* wide_trav;
+    uint32_t* kids;            // 8 child clusters per node (kNone past the last)
 };
 
 // One level of the collapse in one launch.  Each thread takes one node of the
@@ -321,40 +371,13 @@ __global__ __launch_bounds__(kThreads) void k_wide_level(LevelArgs a) {
     // the node: children boxes quantised outward against the node's origin
     // the binary traversal node of this node's root cluster (learned order)
     a.wide_trav[level_base + k] = a.bidx ? a.bidx[a.pos[a.items[k]]] : kNone;
-    float lo[kWidth][3], hi[kWidth][3];
-    float org[3] = {INFINITY, INFINITY, INFINITY};
-    double top[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = 0; i < n; ++i) {
-        const rtq_aabb b = a.cbox[ch[i]];
-        for (int q = 0; q < 3; ++q) {
-            lo[i][q] = b.min[q] - a.pad;
-            hi[i][q] = b.max[q] + a.pad;
-            org[q] = fminf(org[q], lo[i][q]);
-            top[q] = fmax(top[q], (double)hi[i][q]);
-        }
-    }
-    int ex[3];
-    double scale[3];
-    for (int q = 0; q < 3; ++q) {
-        ex[q] = exponent_for(top[q] - (double)org[q]);
-        scale[q] = ldexp(1.0, ex[q]);
-    }
-    uint32_t qlo[3][2] = {{0, 0}, {0, 0}, {0, 0}}, qhi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    float4* o = a.nodes + 8ull * (level_base + k);
+    write_wide_boxes(ch, n, a.cbox, a.pad, o);
     uint32_t ref[kWidth];
-    uint32_t valid = 0, rank = 0;
+    uint32_t rank = 0;
     for (uint32_t i = 0; i < kWidth; ++i) {
         ref[i] = 0u;
         if (i >= n) continue;
-        valid |= 1u << i;
-        for (int q = 0; q < 3; ++q) {
-            // exact: float differences and division by a power of two are exact in double
-            double l = floor(((double)lo[i][q] - (double)org[q]) / scale[q]);
-            double h = ceil(((double)hi[i][q] - (double)org[q]) / scale[q]);
-            l = fmin(fmax(l, 0.0), 255.0);
-            h = fmin(fmax(h, 0.0), 255.0);
-            qlo[q][i >> 2] |= (uint32_t)l << (8 * (i & 3));
-            qhi[q][i >> 2] |= (uint32_t)h << (8 * (i & 3));
-        }
         const uint32_t c = ch[i];
         if ((expand >> i) & 1u) {
             const uint32_t slot = excl + local + rank++;
@@ -368,22 +391,34 @@ __global__ __launch_bounds__(kThreads) void k_wide_level(LevelArgs a) {
             ref[i] = leaf_ref(a.fl[c], a.ccount[c]);
         }
     }
-    const uint32_t meta = (uint32_t)(ex[0] + 127) | ((uint32_t)(ex[1] + 127) << 8) |
-                          ((uint32_t)(ex[2] + 127) << 16) | (valid << 24);
-    float4* o = a.nodes + 8ull * (level_base + k);
-    o[0] = make_float4(org[0], org[1], org[2], __uint_as_float(meta));
-    o[1] = make_float4(__uint_as_float(qlo[0][0]), __uint_as_float(qlo[0][1]), __uint_as_float(qhi[0][0]),
-                       __uint_as_float(qhi[0][1]));
-    o[2] = make_float4(__uint_as_float(qlo[1][0]), __uint_as_float(qlo[1][1]), __uint_as_float(qhi[1][0]),
-                       __uint_as_float(qhi[1][1]));
-    o[3] = make_float4(__uint_as_float(qlo[2][0]), __uint_as_float(qlo[2][1]), __uint_as_float(qhi[2][0]),
-                       __uint_as_float(qhi[2][1]));
     o[4] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]),
                        __uint_as_float(ref[3]));
     o[5] = make_float4(__uint_as_float(ref[4]), __uint_as_float(ref[5]), __uint_as_float(ref[6]),
                        __uint_as_float(ref[7]));
     o[6] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // the node's child clusters, for a refit
+    uint4* kid = reinterpret_cast<uint4*>(a.kids + 8ull * (level_base + k));
+    kid[0] = make_uint4(ch[0], n > 1 ? ch[1] : kNone, n > 2 ? ch[2] : kNone, n > 3 ? ch[3] : kNone);
+    kid[1] = make_uint4(n > 4 ? ch[4] : kNone, n > 5 ? ch[5] : kNone, n > 6 ? ch[6] : kNone, n > 7 ? ch[7] : kNone);
+}
+
+// New boxes for the kept 8-wide nodes after a refit: each node's child
+// clusters (kids, kNone past the last) quantised again from their refitted
+// boxes; child refs stay, learned tables are cleared.
+__global__ __launch_bounds__(kThreads) void k_wide_refit(uint32_t n_wide, const uint32_t* __restrict__ kids,
+                                                         const rtq_aabb* __restrict__ cbox, float pad,
+                                                         float4* __restrict__ nodes) {
+    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= n_wide) return;
+    const uint4* kp = reinterpret_cast<const uint4*>(kids + 8ull * k);
+    const uint4 k0 = kp[0], k1 = kp[1];
+    const uint32_t ch[kWidth] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+    uint32_t n = 0;
+    while (n < kWidth && ch[n] != kNone) ++n;
+    float4* o = nodes + 8ull * k;
+    write_wide_boxes(ch, n, cbox, pad, o);
+    o[6] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
@@ -437,6 +472,7 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint
     // a refit writes the same buffer again (its size depends on the topology only)
     if (!s->wide) RTQ_HIP(ctx, take(s->kept_wide, &s->wide, 8ull * cap));
     if (!s->wide_trav) RTQ_HIP(ctx, take(s->kept_wide_trav, &s->wide_trav, cap));
+    if (!s->wide_kids) RTQ_HIP(ctx, take(s->kept_wide_kids, &s->wide_kids, 8ull * cap));
     s->wide_ordered = false;   // the nodes are rewritten with empty tables
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, items, root);
     RTQ_HIP(ctx, hipGetLastError());
@@ -467,6 +503,7 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint
     la.bidx = bidx;
     la.nodes = s->wide;
     la.wide_trav = s->wide_trav;
+    la.kids = s->wide_kids;
     uint32_t level_items = 1, level_base = 0, level_status = 0, levels = 0, n_wide = 0;
     while (level_items > 0) {
         // the group's first level; the others (and the error word) zero until written
@@ -502,6 +539,16 @@ rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint
     }
     s->n_wide = n_wide;
     s->wide_depth = levels;
+    return RTQ_OK;
+}
+
+rtq_status refit_wide(rtq_ctx* ctx, Scene* s, hipStream_t st) {
+    if (!s->wide || !s->wide_kids) return fail(ctx, RTQ_EINTERNAL, "refit: the scene has no 8-wide nodes");
+    if (s->n_wide == 0) return RTQ_OK;
+    hipLaunchKernelGGL(k_wide_refit, dim3(blocks_for(s->n_wide)), dim3(kThreads), 0, st, s->n_wide, s->wide_kids,
+                       s->cbox, s->box_pad, s->wide);
+    RTQ_HIP(ctx, hipGetLastError());
+    s->wide_ordered = false;   // the tables were cleared
     return RTQ_OK;
 }
 

@@ -58,11 +58,14 @@ def test_refit_unchanged_is_the_build(rtq_ctx, n, width):
     try:
         info0 = sc.build(16, 4, 0, node_width=width)
         ref0, (trav0, rng0), cl0 = sc.export_ref(), sc.export_trav(), sc.export_clusters()
+        wide0 = sc.export_wide() if width == 8 else None
         rays = rays_for()
         h0 = sc.trace_host(rays)
         sc.update(tris)
         sc.refit()
         info1 = sc.info()
+        if width == 8:   # the refit writes the 8-wide nodes the build wrote, bit for bit
+            assert np.array_equal(sc.export_wide(), wide0)
         assert hits_equal(sc.export_ref(), ref0)
         trav1, rng1 = sc.export_trav()
         assert np.array_equal(trav1.view(np.uint32), trav0.view(np.uint32)) and np.array_equal(rng1, rng0)
@@ -82,8 +85,9 @@ def test_refit_moved_vertices(rtq_ctx, width):
     models = scenes.identity_models()
     sc = rtq_ctx.upload(tris, models)
     try:
-        sc.build(16, 4, 0, node_width=width)
+        info0 = sc.build(16, 4, 0, node_width=width)
         cl = sc.export_clusters()
+        wide0 = sc.export_wide() if width == 8 else None
         new = moved(tris, amount=0.05)
         sc.update(new)
         with pytest.raises(RtqError) as e:
@@ -97,11 +101,15 @@ def test_refit_moved_vertices(rtq_ctx, width):
         assert all(np.array_equal(got[k], cl[k]) for k in ("triangle_id", "left", "right", "parent"))
         ref = oracle.flatten_preorder(n, dict(cl, boxes=boxes))
         assert hits_equal(sc.export_ref(), ref), first_diff(sc.export_ref(), ref)
-        # the 8-wide collapse is recomputed from the new boxes: its node count
-        # and depth may change, within the binary traversal tree's
+        # the 8-wide nodes keep their children (valid masks and child refs)
+        # and get new boxes
         info = sc.info()
         if width == 8:
-            assert 0 < info.n_wide_nodes <= info.n_trav_nodes and 0 < info.wide_depth <= info.depth + 1
+            assert (info.n_wide_nodes, info.wide_depth) == (info0.n_wide_nodes, info0.wide_depth)
+            wide1 = sc.export_wide()
+            assert np.array_equal(wide1[:, 16:24], wide0[:, 16:24])
+            assert np.array_equal(wide1[:, 3] >> 24, wide0[:, 3] >> 24)
+            assert not np.array_equal(wide1[:, :16], wide0[:, :16])
         # hits equal brute force over the new vertices, closest and any hit
         rays = rays_for()
         v9 = oracle.world_vertices(new, models)

@@ -72,7 +72,7 @@ NODE_BYTES = {"binary": BINARY_NODE_BYTES, "wide": WIDE_NODE_BYTES, "learned": B
 # sources of the traversal kernels; the committed traffic summary records
 # their digest and the fingerprint of the terrain tree it was measured on
 # (tree_digest), and is ignored when either differs
-KERNEL_SOURCES = ("trace.hip", "batch.hip", "rtq_internal.hpp")
+KERNEL_SOURCES = ("trace.hip", "batch.hip", "trace_layout.hpp")
 
 
 def kernel_label(rocprof_name: str):

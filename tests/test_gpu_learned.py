@@ -470,6 +470,21 @@ def test_learned_wide_order(rtq_ctx, occ, torch_mod):
         assert not wsc.export_wide()[:, 24:26].any()
         with pytest.raises(RtqError):
             trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED)
+        # a refit keeps the 8-wide nodes and empties their tables; the order
+        # applied again writes them from the kept node mapping
+        wsc.order_apply(model.params.data_ptr(), 0.5, 0.5)
+        wsc.update(occ["tris"])
+        wsc.refit()
+        assert not wsc.export_wide()[:, 24:26].any()
+        with pytest.raises(RtqError):
+            trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED)
+        wsc.order_apply(model.params.data_ptr(), 0.5, 0.5)
+        trav, _ = wsc.export_trav()
+        wide = wsc.export_wide()
+        assert np.array_equal(wide[:, 24:26], wide_order_tables(trav, wide))
+        got_s = trace(wsc, occ["shad"], 4 * n, RTQ_TRACE_ANY | RTQ_TRACE_LEARNED)
+        assert np.array_equal(got_s["id"] >= 0, plain_s["id"] >= 0)
+        assert hits_equal(trace(wsc, occ["diff"], n, RTQ_TRACE_LEARNED), plain_d)
     finally:
         wsc.release()
 

@@ -906,19 +906,69 @@ __global__ __launch_bounds__(kThreads) void k_place_pass(uint32_t first, uint32_
     if (k < count) place_one(first + k, cleft, cright, ccount, pos, fl, dep);
 }
 
-// The same for passes [k0, k1), last pass first (the top of the tree), in one
-// workgroup with a barrier between passes.
-__global__ __launch_bounds__(kSmallThreads) void k_place_small(const uint2* __restrict__ passes, uint32_t k0,
-                                                               uint32_t k1, const uint32_t* __restrict__ cleft,
-                                                               const uint32_t* __restrict__ cright,
-                                                               const uint32_t* __restrict__ ccount,
-                                                               uint32_t* __restrict__ pos, uint32_t* __restrict__ fl,
-                                                               uint32_t* __restrict__ dep) {
+// The passes [k0, k1) at the top of the tree in one workgroup, last pass
+// first, with the placement state (position, first leaf, depth) of their
+// clusters [c_lo, nc) in LDS: a pass reads and writes LDS only, a barrier
+// between passes.  Children from earlier passes get their state in global
+// memory; the clusters of the range write theirs back at the end.  Each
+// thread keeps the child ids and left-child triangle counts of its clusters
+// c_lo + t + kLdsThreads * s in registers.  nc - c_lo <= kLdsClusters.
+__global__ __launch_bounds__(kLdsThreads) void k_place_lds(const uint2* __restrict__ passes, uint32_t k0, uint32_t k1,
+                                                           uint32_t c_lo, uint32_t nc,
+                                                           const uint32_t* __restrict__ cleft,
+                                                           const uint32_t* __restrict__ cright,
+                                                           const uint32_t* __restrict__ ccount,
+                                                           uint32_t* __restrict__ pos, uint32_t* __restrict__ fl,
+                                                           uint32_t* __restrict__ dep) {
+    constexpr int kSlots = kLdsClusters / kLdsThreads;
+    __shared__ uint32_t s_pos[kLdsClusters], s_fl[kLdsClusters], s_dep[kLdsClusters];
+    const uint32_t t = threadIdx.x;
+    const uint32_t R = nc - c_lo;
+    uint32_t L[kSlots], Rc[kSlots], LC[kSlots];
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const uint32_t idx = t + kLdsThreads * q;
+        L[q] = idx < R ? cleft[c_lo + idx] : 0u;
+        Rc[q] = idx < R ? cright[c_lo + idx] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) LC[q] = t + kLdsThreads * q < R ? ccount[L[q]] : 0u;
+    if (t == 0) {  // the root, the last cluster
+        s_pos[R - 1] = pos[nc - 1];
+        s_fl[R - 1] = fl[nc - 1];
+        s_dep[R - 1] = dep[nc - 1];
+    }
+    __syncthreads();
     for (uint32_t k = k1; k > k0; --k) {
         const uint2 p = passes[k - 1];
-        for (uint32_t i = threadIdx.x; i < p.y; i += kSmallThreads)
-            place_one(p.x + i, cleft, cright, ccount, pos, fl, dep);
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            const uint32_t idx = t + kLdsThreads * q, c = c_lo + idx;
+            if (idx >= R || c < p.x || c >= p.x + p.y) continue;
+            const uint32_t pp = s_pos[idx], pf = s_fl[idx], pd = s_dep[idx];
+            const uint32_t l = L[q], r = Rc[q], lc = LC[q];
+            // left child right after the parent, right child after the left subtree
+            if (l >= c_lo) {
+                s_pos[l - c_lo] = pp + 1; s_fl[l - c_lo] = pf; s_dep[l - c_lo] = pd + 1;
+            } else {
+                pos[l] = pp + 1; fl[l] = pf; dep[l] = pd + 1;
+            }
+            if (r >= c_lo) {
+                s_pos[r - c_lo] = pp + 2 * lc; s_fl[r - c_lo] = pf + lc; s_dep[r - c_lo] = pd + 1;
+            } else {
+                pos[r] = pp + 2 * lc; fl[r] = pf + lc; dep[r] = pd + 1;
+            }
+        }
         __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const uint32_t idx = t + kLdsThreads * q;
+        if (idx < R) {
+            pos[c_lo + idx] = s_pos[idx];
+            fl[c_lo + idx] = s_fl[idx];
+            dep[c_lo + idx] = s_dep[idx];
+        }
     }
 }
 
@@ -1320,17 +1370,17 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     if (!refit) {
         hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, root, pos, fl, dep);
         RTQ_HIP(ctx, hipGetLastError());
-        // the small passes at the top of the tree in one launch, then one
-        // launch per larger pass, last pass first
+        // the passes at the top of the tree in one launch (their clusters'
+        // state in LDS), then one launch per larger pass, last pass first
         const uint32_t n_passes = (uint32_t)s->passes.size();
-        if (s->small_from < n_passes) {
-            hipLaunchKernelGGL(k_place_small, dim3(1), dim3(kSmallThreads), 0, st, s->d_passes, s->small_from,
-                               n_passes, s->cleft, s->cright, s->ccount, pos, fl, dep);
+        if (s->lds_from < n_passes) {
+            hipLaunchKernelGGL(k_place_lds, dim3(1), dim3(kLdsThreads), 0, st, s->d_passes, s->lds_from, n_passes,
+                               s->passes[s->lds_from].first, nc, s->cleft, s->cright, s->ccount, pos, fl, dep);
             RTQ_HIP(ctx, hipGetLastError());
         }
         rs = queue_cost(2);
         if (rs != RTQ_OK) return rs;
-        for (uint32_t k = s->small_from; k > 0; --k) {
+        for (uint32_t k = s->lds_from; k > 0; --k) {
             const auto& pr = s->passes[k - 1];
             hipLaunchKernelGGL(k_place_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first,
                                pr.second, s->cleft, s->cright, s->ccount, pos, fl, dep);
@@ -1501,6 +1551,8 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     // merge at most kSmallPass clusters each
     s->small_from = s->ploc_passes;
     while (s->small_from > 0 && s->passes[s->small_from - 1].second <= kSmallPass) --s->small_from;
+    s->lds_from = s->ploc_passes;
+    while (s->lds_from > 0 && nc - s->passes[s->lds_from - 1].first <= kLdsClusters) --s->lds_from;
     if (s->ploc_passes > 0) {
         s->h_passes.resize(s->ploc_passes);
         for (uint32_t k = 0; k < s->ploc_passes; ++k)

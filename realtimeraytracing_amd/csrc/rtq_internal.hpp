@@ -76,10 +76,13 @@ struct Scene {
     // (first cluster id, count) of the merges of each PLOC pass, in build order
     std::vector<std::pair<uint32_t, uint32_t>> passes;
     // the same on the device (and its host staging copy); passes from
-    // small_from on are small enough to run in one workgroup
+    // small_from on merge few enough clusters each to run in one workgroup
+    // (refit merges), and the clusters of the passes from lds_from on fit in
+    // one workgroup's LDS together (kLdsClusters: placement, collapse costs)
     uint2* d_passes = nullptr;
     std::vector<uint2> h_passes;
     uint32_t small_from = 0;
+    uint32_t lds_from = 0;
     // the memory behind nodes, node_aux, wide, wide_trav, wide_kids and
     // d_passes, kept from build to build
     Kept kept_nodes, kept_aux, kept_wide, kept_wide_trav, kept_wide_kids, kept_passes;
@@ -331,6 +334,11 @@ void scratch_free(TraceScratch* sc);
 // one-triangle scene.
 rtq_status build_wide(rtq_ctx* ctx, Scene* s, const uint32_t* choice, const uint32_t* fl, const uint32_t* pos,
                       const uint32_t* bidx, hipStream_t st);
+// Clusters created by the passes from Scene::lds_from on: at most this many
+// (their placement state, 12 B each, and cost rows, 32 B each, in LDS).
+constexpr uint32_t kLdsClusters = 4096;
+constexpr int kLdsThreads = 1024;   // 4 clusters per thread
+
 // The SAH cost table of the 8-wide collapse (cost: 8 floats, choice: one word
 // per internal cluster), bottom-up over the PLOC passes in wide_cost_steps(s)
 // launches; wide_cost_table queues launches [first, last) of them.  It reads

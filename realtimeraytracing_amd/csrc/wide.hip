@@ -66,21 +66,16 @@ __device__ __forceinline__ void dp_row(uint32_t x, uint32_t nt, const float* __r
     }
 }
 
-__device__ __forceinline__ void dp_cluster(uint32_t c, uint32_t nt, uint32_t max_leaf,
-                                           const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
-                                           const uint32_t* __restrict__ ccount, const rtq_aabb* __restrict__ cbox,
-                                           float* __restrict__ cost, uint32_t* __restrict__ choice) {
-    float dl[kWidth], dr[kWidth], row[kWidth];
-    dp_row(cleft[c], nt, cost, cbox, dl);
-    dp_row(cright[c], nt, cost, cbox, dr);
+// Row and choice word of a cluster of `cnt` triangles and box area a from its
+// children's rows dl, dr.
+__device__ __forceinline__ uint32_t dp_combine(const float* dl, const float* dr, float a, uint32_t cnt,
+                                               uint32_t max_leaf, float* row) {
     float bn = INFINITY;
     uint32_t kn = 1;
     for (uint32_t k = 1; k < kWidth; ++k) {
         const float v = dl[k - 1] + dr[kWidth - k - 1];
         if (v < bn) { bn = v; kn = k; }
     }
-    const float a = box_area(cbox[c]);
-    const uint32_t cnt = ccount[c];
     row[0] = cnt <= max_leaf ? kTriCost * a * (float)cnt : kNodeCost * a + bn;
     uint32_t ch = (kn - 1) << 21;
     for (uint32_t j = 2; j <= kWidth; ++j) {
@@ -93,6 +88,17 @@ __device__ __forceinline__ void dp_cluster(uint32_t c, uint32_t nt, uint32_t max
         row[j - 1] = best;
         ch |= pick << (3 * (j - 2));
     }
+    return ch;
+}
+
+__device__ __forceinline__ void dp_cluster(uint32_t c, uint32_t nt, uint32_t max_leaf,
+                                           const uint32_t* __restrict__ cleft, const uint32_t* __restrict__ cright,
+                                           const uint32_t* __restrict__ ccount, const rtq_aabb* __restrict__ cbox,
+                                           float* __restrict__ cost, uint32_t* __restrict__ choice) {
+    float dl[kWidth], dr[kWidth], row[kWidth];
+    dp_row(cleft[c], nt, cost, cbox, dl);
+    dp_row(cright[c], nt, cost, cbox, dr);
+    const uint32_t ch = dp_combine(dl, dr, box_area(cbox[c]), ccount[c], max_leaf, row);
     float* out = cost + (size_t)(c - nt) * kWidth;
 #pragma unroll
     for (int j = 0; j < kWidth; ++j) out[j] = row[j];
@@ -109,19 +115,56 @@ __global__ __launch_bounds__(kThreads) void k_dp_pass(uint32_t first, uint32_t c
     if (i < count) dp_cluster(first + i, nt, max_leaf, cleft, cright, ccount, cbox, cost, choice);
 }
 
-// Passes [k0, k1) in one workgroup, a barrier between passes.
-constexpr int kDpSmallThreads = 1024;
-__global__ __launch_bounds__(kDpSmallThreads) void k_dp_small(const uint2* __restrict__ passes, uint32_t k0,
-                                                              uint32_t k1, uint32_t nt, uint32_t max_leaf,
-                                                              const uint32_t* __restrict__ cleft,
-                                                              const uint32_t* __restrict__ cright,
-                                                              const uint32_t* __restrict__ ccount,
-                                                              const rtq_aabb* __restrict__ cbox,
-                                                              float* __restrict__ cost, uint32_t* __restrict__ choice) {
+// Passes [k0, k1) at the top of the tree in one workgroup, bottom-up, with
+// the cost rows of their clusters [c_lo, nc) in LDS (a child from an earlier
+// pass gives its row from global memory); a barrier between passes.  Each
+// thread keeps the child ids, box area and triangle count of its clusters
+// c_lo + t + kLdsThreads * s in registers.  nc - c_lo <= kLdsClusters.
+__global__ __launch_bounds__(kLdsThreads) void k_dp_lds(const uint2* __restrict__ passes, uint32_t k0, uint32_t k1,
+                                                        uint32_t c_lo, uint32_t nc, uint32_t nt, uint32_t max_leaf,
+                                                        const uint32_t* __restrict__ cleft,
+                                                        const uint32_t* __restrict__ cright,
+                                                        const uint32_t* __restrict__ ccount,
+                                                        const rtq_aabb* __restrict__ cbox, const float* __restrict__ cost,
+                                                        uint32_t* __restrict__ choice) {
+    constexpr int kSlots = kLdsClusters / kLdsThreads;
+    __shared__ float s_row[kLdsClusters * kWidth];
+    const uint32_t t = threadIdx.x;
+    const uint32_t R = nc - c_lo;
+    uint32_t L[kSlots], Rc[kSlots], N[kSlots];
+    float A[kSlots];
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const uint32_t idx = t + kLdsThreads * q;
+        const bool in = idx < R;
+        L[q] = in ? cleft[c_lo + idx] : 0u;
+        Rc[q] = in ? cright[c_lo + idx] : 0u;
+        N[q] = in ? ccount[c_lo + idx] : 0u;
+        A[q] = in ? box_area(cbox[c_lo + idx]) : 0.0f;
+    }
     for (uint32_t k = k0; k < k1; ++k) {
         const uint2 p = passes[k];
-        for (uint32_t i = threadIdx.x; i < p.y; i += kDpSmallThreads)
-            dp_cluster(p.x + i, nt, max_leaf, cleft, cright, ccount, cbox, cost, choice);
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            const uint32_t idx = t + kLdsThreads * q, c = c_lo + idx;
+            if (idx >= R || c < p.x || c >= p.x + p.y) continue;
+            float dl[kWidth], dr[kWidth], row[kWidth];
+            if (L[q] >= c_lo) {
+#pragma unroll
+                for (int j = 0; j < kWidth; ++j) dl[j] = s_row[(L[q] - c_lo) * kWidth + j];
+            } else {
+                dp_row(L[q], nt, cost, cbox, dl);
+            }
+            if (Rc[q] >= c_lo) {
+#pragma unroll
+                for (int j = 0; j < kWidth; ++j) dr[j] = s_row[(Rc[q] - c_lo) * kWidth + j];
+            } else {
+                dp_row(Rc[q], nt, cost, cbox, dr);
+            }
+            choice[c - nt] = dp_combine(dl, dr, A[q], N[q], max_leaf, row);
+#pragma unroll
+            for (int j = 0; j < kWidth; ++j) s_row[idx * kWidth + j] = row[j];
+        }
         __syncthreads();
     }
 }
@@ -429,7 +472,7 @@ inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kThreads - 1) / 
 
 uint32_t wide_cost_steps(const Scene* s) {
     if (s->n_clusters <= s->n_tris) return 0;
-    return s->small_from + (s->small_from < (uint32_t)s->passes.size() ? 1u : 0u);
+    return s->lds_from + (s->lds_from < (uint32_t)s->passes.size() ? 1u : 0u);
 }
 
 rtq_status wide_cost_table(rtq_ctx* ctx, const Scene* s, float* cost, uint32_t* choice, uint32_t first,
@@ -437,13 +480,14 @@ rtq_status wide_cost_table(rtq_ctx* ctx, const Scene* s, float* cost, uint32_t* 
     const uint32_t nt = s->n_tris, max_leaf = s->params.max_leaf_size;
     const uint32_t n_passes = (uint32_t)s->passes.size();
     for (uint32_t k = first; k < last; ++k) {
-        if (k < s->small_from) {
+        if (k < s->lds_from) {
             const auto& pr = s->passes[k];
             hipLaunchKernelGGL(k_dp_pass, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
                                nt, max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
         } else {
-            hipLaunchKernelGGL(k_dp_small, dim3(1), dim3(kDpSmallThreads), 0, st, s->d_passes, s->small_from,
-                               n_passes, nt, max_leaf, s->cleft, s->cright, s->ccount, s->cbox, cost, choice);
+            hipLaunchKernelGGL(k_dp_lds, dim3(1), dim3(kLdsThreads), 0, st, s->d_passes, s->lds_from, n_passes,
+                               s->passes[s->lds_from].first, s->n_clusters, nt, max_leaf, s->cleft, s->cright,
+                               s->ccount, s->cbox, cost, choice);
         }
         RTQ_HIP(ctx, hipGetLastError());
     }

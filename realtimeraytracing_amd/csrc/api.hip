@@ -60,7 +60,6 @@ void reset_build(Scene* s) {
     s->wide_kids = nullptr;
     s->wide_ordered = false;
     s->d_passes = nullptr;
-    s->small_from = 0;
     s->lds_from = 0;
     s->n_top = 0;
     s->built = false;

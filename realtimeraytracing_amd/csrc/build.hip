@@ -90,21 +90,47 @@ __global__ __launch_bounds__(kThreads) void k_refit_merge(uint32_t first, uint32
     if (i < count) refit_merge_one(first + i, cleft, cright, cbox);
 }
 
-// Passes that merge few clusters (the top of the tree) would each be a tiny
-// launch: one workgroup runs passes [k0, k1) of `passes` (first id, merges)
-// in order, a barrier between passes (a pass reads boxes the previous ones
-// wrote, which the workgroup's own stores and barrier make visible).
-constexpr uint32_t kSmallPass = 4096;
-constexpr int kSmallThreads = 1024;
-
-__global__ __launch_bounds__(kSmallThreads) void k_refit_merge_small(const uint2* __restrict__ passes, uint32_t k0,
-                                                                     uint32_t k1, const uint32_t* __restrict__ cleft,
-                                                                     const uint32_t* __restrict__ cright,
-                                                                     rtq_aabb* __restrict__ cbox) {
+// The passes [k0, k1) at the top of the tree in one workgroup, in build order,
+// with the boxes of their clusters [c_lo, nc) in LDS (a child from an earlier
+// pass gives its box from global memory); a barrier between passes, and the
+// range's boxes written back at the end.  nc - c_lo <= kLdsClusters.
+__global__ __launch_bounds__(kLdsThreads) void k_refit_merge_lds(const uint2* __restrict__ passes, uint32_t k0,
+                                                                 uint32_t k1, uint32_t c_lo, uint32_t nc,
+                                                                 const uint32_t* __restrict__ cleft,
+                                                                 const uint32_t* __restrict__ cright,
+                                                                 rtq_aabb* __restrict__ cbox) {
+    constexpr int kSlots = kLdsClusters / kLdsThreads;
+    __shared__ rtq_aabb s_box[kLdsClusters];
+    const uint32_t t = threadIdx.x;
+    const uint32_t R = nc - c_lo;
+    uint32_t L[kSlots], Rc[kSlots];
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const uint32_t idx = t + kLdsThreads * q;
+        L[q] = idx < R ? cleft[c_lo + idx] : 0u;
+        Rc[q] = idx < R ? cright[c_lo + idx] : 0u;
+    }
     for (uint32_t k = k0; k < k1; ++k) {
         const uint2 p = passes[k];
-        for (uint32_t i = threadIdx.x; i < p.y; i += kSmallThreads) refit_merge_one(p.x + i, cleft, cright, cbox);
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            const uint32_t idx = t + kLdsThreads * q, c = c_lo + idx;
+            if (idx >= R || c < p.x || c >= p.x + p.y) continue;
+            const rtq_aabb l = L[q] >= c_lo ? s_box[L[q] - c_lo] : cbox[L[q]];
+            const rtq_aabb r = Rc[q] >= c_lo ? s_box[Rc[q] - c_lo] : cbox[Rc[q]];
+            rtq_aabb m{};
+            for (int d = 0; d < 3; ++d) {   // the merge of refit_merge_one
+                m.min[d] = smin(l.min[d], r.min[d]);
+                m.max[d] = smax(l.max[d], r.max[d]);
+            }
+            s_box[idx] = m;
+        }
         __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const uint32_t idx = t + kLdsThreads * q;
+        if (idx < R) cbox[c_lo + idx] = s_box[idx];
     }
 }
 
@@ -1546,11 +1572,9 @@ rtq_status build_scene(rtq_ctx* ctx, Scene* s, const rtq_build_params& prm) {
     if (rs != RTQ_OK) return rs;
     s->ploc_passes = (uint32_t)pass_ranges.size();
     s->passes = pass_ranges;
-    // device copy of the pass ranges for the single-launch small passes
-    // (placement here, box merges in a refit): the passes from small_from on
-    // merge at most kSmallPass clusters each
-    s->small_from = s->ploc_passes;
-    while (s->small_from > 0 && s->passes[s->small_from - 1].second <= kSmallPass) --s->small_from;
+    // device copy of the pass ranges for the single-workgroup launches of the
+    // top passes (placement, collapse costs, refit merges): the passes from
+    // lds_from on create at most kLdsClusters clusters in all
     s->lds_from = s->ploc_passes;
     while (s->lds_from > 0 && nc - s->passes[s->lds_from - 1].first <= kLdsClusters) --s->lds_from;
     if (s->ploc_passes > 0) {
@@ -1614,15 +1638,15 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
                        n, s->cbox);
     RTQ_HIP(ctx, hipGetLastError());
     const uint32_t n_passes = (uint32_t)s->passes.size();
-    for (uint32_t k = 0; k < s->small_from; ++k) {
+    for (uint32_t k = 0; k < s->lds_from; ++k) {
         const auto& pr = s->passes[k];
         hipLaunchKernelGGL(k_refit_merge, dim3(blocks_for(pr.second)), dim3(kThreads), 0, st, pr.first, pr.second,
                            s->cleft, s->cright, s->cbox);
         RTQ_HIP(ctx, hipGetLastError());
     }
-    if (s->small_from < n_passes) {
-        hipLaunchKernelGGL(k_refit_merge_small, dim3(1), dim3(kSmallThreads), 0, st, s->d_passes, s->small_from,
-                           n_passes, s->cleft, s->cright, s->cbox);
+    if (s->lds_from < n_passes) {
+        hipLaunchKernelGGL(k_refit_merge_lds, dim3(1), dim3(kLdsThreads), 0, st, s->d_passes, s->lds_from, n_passes,
+                           s->passes[s->lds_from].first, s->n_clusters, s->cleft, s->cright, s->cbox);
         RTQ_HIP(ctx, hipGetLastError());
     }
     rtq_status rs = compute_bounds(ctx, s, s->params.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);

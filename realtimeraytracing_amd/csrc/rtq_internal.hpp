@@ -75,13 +75,12 @@ struct Scene {
     uint32_t ploc_passes = 0;
     // (first cluster id, count) of the merges of each PLOC pass, in build order
     std::vector<std::pair<uint32_t, uint32_t>> passes;
-    // the same on the device (and its host staging copy); passes from
-    // small_from on merge few enough clusters each to run in one workgroup
-    // (refit merges), and the clusters of the passes from lds_from on fit in
-    // one workgroup's LDS together (kLdsClusters: placement, collapse costs)
+    // the same on the device (and its host staging copy); the clusters of the
+    // passes from lds_from on fit in one workgroup's LDS together
+    // (kLdsClusters): placement, collapse costs and refit merges run those
+    // passes in one launch
     uint2* d_passes = nullptr;
     std::vector<uint2> h_passes;
-    uint32_t small_from = 0;
     uint32_t lds_from = 0;
     // the memory behind nodes, node_aux, wide, wide_trav, wide_kids and
     // d_passes, kept from build to build

@@ -51,7 +51,7 @@ def rays_for(n=4096, seed=3):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,width", [(1, 2), (5000, 2), (20000, 8)])
+@pytest.mark.parametrize("n,width", [(1, 2), (1, 8), (1500, 8), (5000, 2), (20000, 8)])
 def test_refit_unchanged_is_the_build(rtq_ctx, n, width):
     tris = scenes.soup(n, seed=n)
     sc = rtq_ctx.upload(tris, scenes.identity_models())

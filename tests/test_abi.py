@@ -40,6 +40,23 @@ def test_exports_match_header():
     assert {e for e in exported if not e.startswith("_")} == declared   # nothing else leaks
 
 
+def test_probe_library_exports_its_entry_point():
+    """The vector-memory probe bench.py measures with is its own library: it
+    exports one entry point, and librtq does not carry it."""
+    probe = ROOT / "realtimeraytracing_amd" / "librtqprobe.so"
+    assert probe.exists(), "build first"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(probe)], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert {e for e in exported if not e.startswith("_")} == {"rtqprobe_vmem_rate"}
+    lib = ctypes.CDLL(str(probe))
+    fn = lib.rtqprobe_vmem_rate
+    fn.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    fn.restype = ctypes.c_int
+    r = ctypes.c_double(0.0)
+    assert fn(0, 9, 16, 6, ctypes.byref(r)) != 0    # 9 loads per step: refused before any device call
+    assert fn(0, 6, 16, 6, None) != 0
+
+
 def test_numpy_layouts():
     assert TRIANGLE_DTYPE.itemsize == 64 and TRIANGLE_DTYPE.fields["model_id"][1] == 48
     assert MODEL_DTYPE.itemsize == 68

@@ -251,3 +251,13 @@ def test_stats_counters(rtq_ctx, terrain):
     nt = devmem.to_host(d_nt, np.uint32)
     assert (nv >= 1).all() and (nt >= 1).all()
     assert nv.max() < terrain.info.n_trav_nodes
+
+
+def test_vmem_probe_rates():
+    """The dependent-load probe bench.py prices the trace kernels against:
+    per-lane 16-B loads that hit L1 go faster than loads that miss it, and
+    both rates are in the range of one CU clock per load or two."""
+    import bench
+    r = bench.probe_vmem()
+    assert r is not None
+    assert 1e11 < r["l1_miss"] < r["l1_hit"] < 5e12, r

@@ -272,22 +272,25 @@ VMEM_OPS_PER_RAY = 3        # two 16-B ray loads, one 16-B hit store
 def probe_vmem(slots_miss: int = 219_000, slots_hit: int = 64) -> dict | None:
     """Per-lane 16-B load rates of dependent chains (librtqprobe.so, six loads
     per 128-B slot as an 8-wide visit, 6 workgroups of 256 lanes per CU):
-    over 28 MB of slots (loads miss L1) and over 64 slots (loads hit L1)."""
+    over 28 MB of slots (loads miss L1) and over 64 slots (loads hit L1), with
+    all 64 lanes of each wave loading and (`_32`) with lanes 0-31 only."""
     import ctypes
     lib_path = ROOT / "realtimeraytracing_amd" / "librtqprobe.so"
     if not lib_path.exists():
         return None
     lib = ctypes.CDLL(str(lib_path))
-    fn = lib.rtqprobe_vmem_rate
-    fn.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    fn = lib.rtqprobe_vmem_rate_lanes
+    fn.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                   ctypes.POINTER(ctypes.c_double)]
     fn.restype = ctypes.c_int
     out = {}
-    for name, slots in (("l1_miss", slots_miss), ("l1_hit", slots_hit)):
-        r = ctypes.c_double(0.0)
-        rc = fn(0, 6, slots, 6, ctypes.byref(r))
-        if rc != 0:
-            raise RuntimeError(f"vector-memory probe failed: HIP error {rc}")
-        out[name] = r.value
+    for lanes, suffix in ((64, ""), (32, "_32")):
+        for name, slots in (("l1_miss", slots_miss), ("l1_hit", slots_hit)):
+            r = ctypes.c_double(0.0)
+            rc = fn(0, 6, slots, 6, lanes, ctypes.byref(r))
+            if rc != 0:
+                raise RuntimeError(f"vector-memory probe failed: HIP error {rc}")
+            out[name + suffix] = r.value
     return out
 
 

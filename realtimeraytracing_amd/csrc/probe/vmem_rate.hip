@@ -7,7 +7,9 @@
 // result.  The rate of these per-lane loads, with thousands of chains in
 // flight, is the ceiling of a kernel whose lanes fetch scattered 16-B words:
 // an array far larger than L1 gives the rate of loads that miss L1, a few
-// slots shared by every lane the rate of loads that hit it.
+// slots shared by every lane the rate of loads that hit it.  With fewer than
+// 64 active lanes per wave (the others skip the chain) the same launch shows
+// what a partially active wave pays per load instruction.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -35,31 +37,41 @@ __global__ void k_fill(uint4* __restrict__ slots, uint64_t words) {
 }
 
 template <int L>
-__global__ __launch_bounds__(kThreads) void k_chain(const uint4* __restrict__ slots, uint32_t n, uint32_t* __restrict__ out) {
-    uint32_t cur = mix32(blockIdx.x * kThreads + threadIdx.x) % n;
+__global__ __launch_bounds__(kThreads) void k_chain(const uint4* __restrict__ slots, uint32_t n, uint32_t active,
+                                                    uint32_t* __restrict__ out) {
     uint32_t acc = 0;
-    for (int s = 0; s < kSteps; ++s) {
-        const uint4* p = slots + 8ull * cur;
-        uint32_t h = 0;
+    if ((threadIdx.x & 63u) < active) {   // lanes 0 .. active-1 of each wave
+        uint32_t cur = mix32(blockIdx.x * kThreads + threadIdx.x) % n;
+        for (int s = 0; s < kSteps; ++s) {
+            const uint4* p = slots + 8ull * cur;
+            uint32_t h = 0;
 #pragma unroll
-        for (int k = 0; k < L; ++k) {
-            const uint4 v = p[k];
-            h += (v.x ^ v.y) + (v.z ^ (v.w + (uint32_t)k));
+            for (int k = 0; k < L; ++k) {
+                const uint4 v = p[k];
+                h += (v.x ^ v.y) + (v.z ^ (v.w + (uint32_t)k));
+            }
+            acc += h;
+            cur = mix32(h + (uint32_t)s) % n;
         }
-        acc += h;
-        cur = mix32(h + (uint32_t)s) % n;
     }
     out[blockIdx.x * kThreads + threadIdx.x] = acc;   // keeps the loads live
 }
 
+using ChainKernel = void (*)(const uint4*, uint32_t, uint32_t, uint32_t*);
+constexpr ChainKernel kChains[8] = {k_chain<1>, k_chain<2>, k_chain<3>, k_chain<4>,
+                                    k_chain<5>, k_chain<6>, k_chain<7>, k_chain<8>};
+
 }  // namespace
 
 // Per-lane 16-B loads per second over n_slots 128-B slots, `loads` (1..8) per
-// step, blocks_per_cu 256-thread workgroups per CU; best of 3 launches.
+// step, blocks_per_cu 256-thread workgroups per CU, active_lanes (1..64) lanes
+// of each wave loading (the rate counts their loads only); best of 3 launches.
 // Returns 0, or a HIP error code.
-extern "C" __attribute__((visibility("default"))) int rtqprobe_vmem_rate(int device, uint32_t loads, uint32_t n_slots,
-                                                                         uint32_t blocks_per_cu, double* rate) {
-    if (!rate || loads < 1 || loads > 8 || n_slots < 1 || blocks_per_cu < 1 || blocks_per_cu > 32)
+extern "C" __attribute__((visibility("default"))) int rtqprobe_vmem_rate_lanes(int device, uint32_t loads,
+                                                                               uint32_t n_slots, uint32_t blocks_per_cu,
+                                                                               uint32_t active_lanes, double* rate) {
+    if (!rate || loads < 1 || loads > 8 || n_slots < 1 || blocks_per_cu < 1 || blocks_per_cu > 32 ||
+        active_lanes < 1 || active_lanes > 64)
         return (int)hipErrorInvalidValue;
     hipError_t e = hipSetDevice(device);
     hipDeviceProp_t prop;
@@ -82,16 +94,7 @@ extern "C" __attribute__((visibility("default"))) int rtqprobe_vmem_rate(int dev
     float best = 0.0f;
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {   // the first launch warms the caches
         e = hipEventRecord(e0, 0);
-        switch (loads) {
-            case 1: hipLaunchKernelGGL(k_chain<1>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            case 2: hipLaunchKernelGGL(k_chain<2>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            case 3: hipLaunchKernelGGL(k_chain<3>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            case 4: hipLaunchKernelGGL(k_chain<4>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            case 5: hipLaunchKernelGGL(k_chain<5>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            case 6: hipLaunchKernelGGL(k_chain<6>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            case 7: hipLaunchKernelGGL(k_chain<7>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-            default: hipLaunchKernelGGL(k_chain<8>, dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, out); break;
-        }
+        hipLaunchKernelGGL(kChains[loads - 1], dim3(blocks), dim3(kThreads), 0, 0, slots, n_slots, active_lanes, out);
         if (e == hipSuccess) e = hipGetLastError();
         if (e == hipSuccess) e = hipEventRecord(e1, 0);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -99,10 +102,16 @@ extern "C" __attribute__((visibility("default"))) int rtqprobe_vmem_rate(int dev
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (e == hipSuccess && rep > 0 && (best == 0.0f || ms < best)) best = ms;
     }
-    if (e == hipSuccess) *rate = (double)blocks * kThreads * kSteps * loads / (best * 1e-3);
+    if (e == hipSuccess) *rate = (double)blocks * (kThreads / 64) * active_lanes * kSteps * loads / (best * 1e-3);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (out) (void)hipFree(out);
     if (slots) (void)hipFree(slots);
     return (int)e;
+}
+
+// The same with every lane of each wave loading.
+extern "C" __attribute__((visibility("default"))) int rtqprobe_vmem_rate(int device, uint32_t loads, uint32_t n_slots,
+                                                                         uint32_t blocks_per_cu, double* rate) {
+    return rtqprobe_vmem_rate_lanes(device, loads, n_slots, blocks_per_cu, 64, rate);
 }

@@ -42,12 +42,12 @@ def test_exports_match_header():
 
 def test_probe_library_exports_its_entry_point():
     """The vector-memory probe bench.py measures with is its own library: it
-    exports one entry point, and librtq does not carry it."""
+    exports its two entry points, and librtq does not carry them."""
     probe = ROOT / "realtimeraytracing_amd" / "librtqprobe.so"
     assert probe.exists(), "build first"
     out = subprocess.run(["nm", "-D", "--defined-only", str(probe)], capture_output=True, text=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
-    assert {e for e in exported if not e.startswith("_")} == {"rtqprobe_vmem_rate"}
+    assert {e for e in exported if not e.startswith("_")} == {"rtqprobe_vmem_rate", "rtqprobe_vmem_rate_lanes"}
     lib = ctypes.CDLL(str(probe))
     fn = lib.rtqprobe_vmem_rate
     fn.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
@@ -55,6 +55,12 @@ def test_probe_library_exports_its_entry_point():
     r = ctypes.c_double(0.0)
     assert fn(0, 9, 16, 6, ctypes.byref(r)) != 0    # 9 loads per step: refused before any device call
     assert fn(0, 6, 16, 6, None) != 0
+    lanes = lib.rtqprobe_vmem_rate_lanes
+    lanes.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                      ctypes.POINTER(ctypes.c_double)]
+    lanes.restype = ctypes.c_int
+    assert lanes(0, 6, 16, 6, 0, ctypes.byref(r)) != 0     # no active lane
+    assert lanes(0, 6, 16, 6, 65, ctypes.byref(r)) != 0    # more lanes than a wave has
 
 
 def test_numpy_layouts():

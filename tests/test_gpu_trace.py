@@ -261,3 +261,7 @@ def test_vmem_probe_rates():
     r = bench.probe_vmem()
     assert r is not None
     assert 1e11 < r["l1_miss"] < r["l1_hit"] < 5e12, r
+    # a wave with only half its lanes loading takes more than half a full
+    # wave's time per load: its per-lane rate is lower, though above half
+    for k in ("l1_miss", "l1_hit"):
+        assert 0.5 * r[k] < r[k + "_32"] < 0.95 * r[k], (k, r)

@@ -64,7 +64,9 @@ rtq_status rtq_synchronize(rtq_ctx* ctx);
  * Vertices are transformed by their model matrix once, here, in the order
  * glm's mat4 * vec4 uses (bvh.cpp:238-240), instead of per intersection
  * (raytracer.glsl:107-109).  n_tris has no 65,536 cap (triangle.hpp:21).
- * Rejects n_tris == 0, model ids >= n_models and non-finite vertices.
+ * Rejects n_tris == 0, model ids >= n_models and vertex coordinates that are
+ * not finite or exceed 2^28 in magnitude (checked on the device after the
+ * copy; the message names the lowest failing triangle).
  * Host arrays; the library copies them. */
 rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_tris,
                             const rtq_mesh_model* models, uint32_t n_models,
@@ -74,9 +76,12 @@ rtq_status rtq_scene_release(rtq_ctx* ctx, uint32_t scene_id);
 /* Per-frame scene edits (no counterpart in the reference, which builds its BVH
  * once when it binds the scene, scene.cpp:110-187): new triangle vertices
  * and/or model matrices, same counts as uploaded (NULL keeps an array), with
- * the upload's checks.  Waits for work queued on the device.  A built scene
- * becomes stale: trace, export and shading calls return RTQ_ENOTBUILT until
- * rtq_bvh_refit or rtq_bvh_build. */
+ * the upload's checks.  Waits for work queued on the device.  New triangles
+ * are copied to a staging array (N x 64 B, allocated on the first such update
+ * and kept with the scene), checked there and only then copied over the
+ * scene's, so a rejected update leaves the scene unchanged and its arrays keep
+ * their addresses.  A built scene becomes stale: trace, export and shading
+ * calls return RTQ_ENOTBUILT until rtq_bvh_refit or rtq_bvh_build. */
 rtq_status rtq_scene_update(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle* tris, uint32_t n_tris,
                             const rtq_mesh_model* models, uint32_t n_models);
 

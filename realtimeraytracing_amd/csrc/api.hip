@@ -130,6 +130,8 @@ static void free_scene(Scene* s) {
     release(s->tris_ref);
     release(s->models);
     release(s->tri_world);
+    release(s->tris_stage);
+    release(s->check_words);
     delete s;
 }
 
@@ -159,6 +161,24 @@ __global__ __launch_bounds__(kThreads) void k_world(const rtq_triangle* __restri
         }
         out[3ull * i + v] = make_float4(o[0], o[1], o[2], v == 0 ? __uint_as_float(i) : 0.0f);
     }
+}
+
+// The upload checks of every triangle: bad[0] becomes the lowest index whose
+// model id is out of range, bad[1] the lowest whose vertex coordinates (w
+// included) are not all finite and at most 2^28 in magnitude; both start at
+// kNone.  A float is finite with |x| <= 2^28 exactly when its bits without the
+// sign are at most those of 2^28 (0x4D800000): infinities and NaNs lie above.
+__global__ __launch_bounds__(kThreads) void k_check_tris(const uint4* __restrict__ tris, uint32_t n,
+                                                         uint32_t n_models, uint32_t* __restrict__ bad) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint4 p0 = tris[4ull * i], p1 = tris[4ull * i + 1], p2 = tris[4ull * i + 2], rest = tris[4ull * i + 3];
+    auto mag = [](uint4 v) {   // largest magnitude bits of the four words
+        return max(max(v.x & 0x7FFFFFFFu, v.y & 0x7FFFFFFFu), max(v.z & 0x7FFFFFFFu, v.w & 0x7FFFFFFFu));
+    };
+    const uint32_t m = max(mag(p0), max(mag(p1), mag(p2)));
+    if (rest.x >= n_models) atomicMin(bad, i);
+    if (m > 0x4D800000u) atomicMin(bad + 1, i);
 }
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
@@ -381,27 +401,38 @@ rtq_status rtq_synchronize(rtq_ctx* ctx) {
 
 // Upload checks (shared with rtq_scene_update): model ids in range, vertices
 // and matrix entries finite and bounded by 2^28 so every box surface area is
-// finite.
-static rtq_status check_scene_input(rtq_ctx* ctx, const char* what, const rtq_triangle* tris, uint32_t n_tris,
-                                    const rtq_mesh_model* models, uint32_t n_models) {
+// finite.  The model matrices are checked here on the host, before anything
+// is copied.
+static rtq_status check_models(rtq_ctx* ctx, const char* what, const rtq_mesh_model* models, uint32_t n_models) {
     const float lim = 268435456.0f;
-    if (models)
-        for (uint32_t m = 0; m < n_models; ++m)
-            for (int k = 0; k < 16; ++k)
-                if (!finite_and_bounded(models[m].m[k], lim))
-                    return fail(ctx, RTQ_EINVAL, "%s: model %u has a non-finite or huge entry", what, m);
-    if (tris)
-        for (uint32_t i = 0; i < n_tris; ++i) {
-            const rtq_triangle& t = tris[i];
-            if (t.model_id >= n_models)
-                return fail(ctx, RTQ_EINVAL, "%s: triangle %u model id %u out of range", what, i, t.model_id);
-            const rtq_vec4* ps[3] = {&t.p0, &t.p1, &t.p2};
-            for (const rtq_vec4* p : ps)
-                if (!finite_and_bounded(p->x, lim) || !finite_and_bounded(p->y, lim) || !finite_and_bounded(p->z, lim) ||
-                    !finite_and_bounded(p->w, lim))
-                    return fail(ctx, RTQ_EINVAL, "%s: triangle %u has a non-finite or huge vertex", what, i);
-        }
+    for (uint32_t m = 0; m < n_models; ++m)
+        for (int k = 0; k < 16; ++k)
+            if (!finite_and_bounded(models[m].m[k], lim))
+                return fail(ctx, RTQ_EINVAL, "%s: model %u has a non-finite or huge entry", what, m);
     return RTQ_OK;
+}
+
+// The triangles are checked on the device after their copy (dev, n of them;
+// host = the caller's array, for the message): one pass over HBM instead of
+// a second pass over host memory.  Waits for the stream; the message names
+// the lowest failing triangle and, as a host loop over the triangles in order
+// would, its model id first.
+static rtq_status check_tris_on_device(rtq_ctx* ctx, const char* what, const rtq_triangle* host,
+                                       const rtq_triangle* dev, uint32_t n, uint32_t n_models, uint32_t* words,
+                                       hipStream_t st) {
+    RTQ_HIP(ctx, hipMemsetAsync(words, 0xFF, 2 * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_check_tris, dim3(blocks_for(n)), dim3(kThreads), 0, st, reinterpret_cast<const uint4*>(dev),
+                       n, n_models, words);
+    RTQ_HIP(ctx, hipGetLastError());
+    uint32_t bad[2];
+    HostReads rd(ctx, st);
+    RTQ_HIP(ctx, rd.add(bad, words, sizeof bad));
+    RTQ_HIP(ctx, rd.wait());
+    const uint32_t i = std::min(bad[0], bad[1]);
+    if (i == kNone) return RTQ_OK;
+    if (bad[0] == i)
+        return fail(ctx, RTQ_EINVAL, "%s: triangle %u model id %u out of range", what, i, host[i].model_id);
+    return fail(ctx, RTQ_EINVAL, "%s: triangle %u has a non-finite or huge vertex", what, i);
 }
 
 rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_tris, const rtq_mesh_model* models,
@@ -411,7 +442,7 @@ rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_t
     if (n_tris == 0) return fail(ctx, RTQ_EINVAL, "scene_upload: no triangles (the reference root id 2N-2 underflows)");
     if (n_models == 0) return fail(ctx, RTQ_EINVAL, "scene_upload: no mesh models");
     if (n_tris > kMaxTris) return fail(ctx, RTQ_ECAPACITY, "scene_upload: %u triangles exceeds %u", n_tris, kMaxTris);
-    rtq_status rs = check_scene_input(ctx, "scene_upload", tris, n_tris, models, n_models);
+    rtq_status rs = check_models(ctx, "scene_upload", models, n_models);
     if (rs != RTQ_OK) return rs;
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
     Scene* s = new Scene();
@@ -420,13 +451,21 @@ rtq_status rtq_scene_upload(rtq_ctx* ctx, const rtq_triangle* tris, uint32_t n_t
     hipError_t e = dev_alloc(&s->tris_ref, n_tris);
     if (e == hipSuccess) e = dev_alloc(&s->models, n_models);
     if (e == hipSuccess) e = dev_alloc(&s->tri_world, 3ull * n_tris);
+    if (e == hipSuccess) e = dev_alloc(&s->check_words, 2);
     if (e == hipSuccess) e = hipMemcpyAsync(s->tris_ref, tris, sizeof(rtq_triangle) * n_tris, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->models, models, sizeof(DevModel) * n_models, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_world, dim3(blocks_for(n_tris)), dim3(kThreads), 0, ctx->stream, s->tris_ref, s->models,
-                           n_tris, s->tri_world);
-        e = hipGetLastError();
+    if (e != hipSuccess) {
+        free_scene(s);
+        return hip_fail(ctx, e, "scene_upload");
     }
+    rs = check_tris_on_device(ctx, "scene_upload", tris, s->tris_ref, n_tris, n_models, s->check_words, ctx->stream);
+    if (rs != RTQ_OK) {
+        free_scene(s);
+        return rs;
+    }
+    hipLaunchKernelGGL(k_world, dim3(blocks_for(n_tris)), dim3(kThreads), 0, ctx->stream, s->tris_ref, s->models,
+                       n_tris, s->tri_world);
+    e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         free_scene(s);
@@ -448,15 +487,26 @@ rtq_status rtq_scene_update(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle*
     if (models && n_models != s->n_models)
         return fail(ctx, RTQ_EINVAL, "scene_update: %u models, the scene has %u", n_models, s->n_models);
     if (!tris && !models) return RTQ_OK;
-    // model ids of kept triangles are checked against the kept model count
-    rtq_status rs = check_scene_input(ctx, "scene_update", tris, s->n_tris, models, s->n_models);
-    if (rs != RTQ_OK) return rs;
+    if (models) {
+        const rtq_status rs = check_models(ctx, "scene_update", models, s->n_models);
+        if (rs != RTQ_OK) return rs;
+    }
     RTQ_HIP(ctx, hipSetDevice(ctx->device));
     // work already queued on the scene may still read the arrays
     RTQ_HIP(ctx, hipDeviceSynchronize());
-    if (tris)
-        RTQ_HIP(ctx, hipMemcpyAsync(s->tris_ref, tris, sizeof(rtq_triangle) * s->n_tris, hipMemcpyHostToDevice,
-                                    ctx->stream));
+    if (tris) {
+        // new triangles go to the staging array first and replace the
+        // scene's only once they pass the checks (model ids against the kept
+        // model count), so a rejected update leaves the scene as it was; the
+        // scene's array keeps its address for work captured on it
+        if (!s->tris_stage) RTQ_HIP(ctx, dev_alloc(&s->tris_stage, s->n_tris));
+        const size_t bytes = sizeof(rtq_triangle) * s->n_tris;
+        RTQ_HIP(ctx, hipMemcpyAsync(s->tris_stage, tris, bytes, hipMemcpyHostToDevice, ctx->stream));
+        const rtq_status rs = check_tris_on_device(ctx, "scene_update", tris, s->tris_stage, s->n_tris, s->n_models,
+                                                   s->check_words, ctx->stream);
+        if (rs != RTQ_OK) return rs;
+        RTQ_HIP(ctx, hipMemcpyAsync(s->tris_ref, s->tris_stage, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     if (models)
         RTQ_HIP(ctx, hipMemcpyAsync(s->models, models, sizeof(DevModel) * s->n_models, hipMemcpyHostToDevice,
                                     ctx->stream));

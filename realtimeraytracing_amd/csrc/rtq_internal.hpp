@@ -38,6 +38,8 @@ struct Scene {
     rtq_triangle* tris_ref = nullptr;   // N x 64 B, as uploaded
     DevModel* models = nullptr;         // n_models x 68 B
     float4* tri_world = nullptr;        // N x 3 float4: world vertices, original order
+    rtq_triangle* tris_stage = nullptr; // N x 64 B: an update's triangles until they pass the checks
+    uint32_t* check_words = nullptr;    // 2 words: first triangle failing each device check
 
     // build products
     bool built = false;

@@ -185,6 +185,66 @@ def test_update_errors(rtq_ctx):
 
 
 @pytest.mark.gpu
+def test_upload_and_update_input_checks(rtq_ctx):
+    """The triangle checks (run on the device after the copy): model ids in
+    range and every vertex word, w included, finite and at most 2^28 in
+    magnitude.  2^28 itself, -0 and subnormals pass; the error names the
+    lowest failing triangle, its model id first when both checks fail; a
+    rejected update leaves the scene's triangles as they were."""
+    models = scenes.identity_models()
+    lim = np.float32(2.0 ** 28)
+    good = scenes.soup(2000, seed=21)
+    edge = good.copy()
+    edge["p0"][10, 0] = lim
+    edge["p1"][11, 1] = -lim
+    edge["p2"][12, 2] = np.float32(-0.0)
+    edge["p0"][13, 3] = np.float32(1e-40)
+    rtq_ctx.upload(edge, models).release()
+
+    def bad(changes):
+        t = good.copy()
+        for name, i, k, val in changes:
+            if name == "model_id":
+                t["model_id"][i] = val
+            else:
+                t[name][i, k] = val
+        return t
+
+    above = np.nextafter(lim, np.float32(np.inf))
+    cases = [
+        (bad([("model_id", 900, 0, 1), ("p1", 1500, 0, np.nan)]), "triangle 900 model id 1 out of range"),
+        (bad([("p2", 1200, 0, above), ("model_id", 1300, 0, 7)]), "triangle 1200 has a non-finite or huge vertex"),
+        (bad([("p0", 5, 2, np.inf), ("model_id", 5, 0, 2)]), "triangle 5 model id 2 out of range"),
+        (bad([("p1", 0, 3, -np.inf)]), "triangle 0 has a non-finite or huge vertex"),
+        (bad([("p2", 1999, 1, -above)]), "triangle 1999 has a non-finite or huge vertex"),
+    ]
+    for tris, msg in cases:
+        with pytest.raises(RtqError) as e:
+            rtq_ctx.upload(tris, models)
+        assert e.value.status == RTQ_EINVAL and msg in str(e.value), (msg, str(e.value))
+
+    sc = rtq_ctx.upload(good, models)
+    try:
+        sc.build(16, 4, 0, node_width=8)
+        rays = rays_for(4096, seed=31)
+        before = sc.trace_host(rays)
+        for tris, msg in cases:
+            with pytest.raises(RtqError) as e:
+                sc.update(tris)
+            assert e.value.status == RTQ_EINVAL and msg in str(e.value), (msg, str(e.value))
+            after = sc.trace_host(rays)                  # still built, same triangles
+            assert np.array_equal(after.view(np.uint8), before.view(np.uint8))
+        new = moved(good, seed=9)
+        sc.update(new)
+        sc.refit()
+        got = sc.trace_host(rays)
+        ref = oracle.trace_brute(oracle.world_vertices(new, models), rays)
+        assert hits_equal(got, ref), first_diff(got, ref)
+    finally:
+        sc.release()
+
+
+@pytest.mark.gpu
 def test_batch_refuses_stale_or_rebuilt_scene(rtq_ctx, torch_mod):
     """A batch graph holds its scenes' node buffers: it replays after a refit
     (buffers kept), and refuses after an edit without refit or a rebuild."""

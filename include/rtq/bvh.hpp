@@ -129,6 +129,14 @@ class BVH {
         fill(_InternalStruct._NbTriangles);
     }
 
+    // The same edit with the new triangles in device memory (n_triangles 64-B
+    // records, e.g. written by an animation kernel): no host copy.
+    void update_device(const rtq_triangle* triangles_dev, uint32_t n_triangles) {
+        ctx_->check(rtq_scene_update_device(ctx_->get(), scene_, triangles_dev, n_triangles));
+        ctx_->check(rtq_bvh_refit(ctx_->get(), scene_));
+        fill(_InternalStruct._NbTriangles);
+    }
+
     // getColor for device hit records into device RGBA32F pixels (rtq_shade).
     void shade(const rtq_hit* hits_dev, uint64_t n, const rtq_material* materials_dev, uint32_t n_materials,
                float* rgba_dev, uint32_t flags = 0, const float* overlay_dev = nullptr, void* stream = nullptr) const {

@@ -85,6 +85,15 @@ rtq_status rtq_scene_release(rtq_ctx* ctx, uint32_t scene_id);
 rtq_status rtq_scene_update(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle* tris, uint32_t n_tris,
                             const rtq_mesh_model* models, uint32_t n_models);
 
+/* rtq_scene_update with the new triangles already in device memory (n_tris
+ * 64-B records at tris_dev on the context's device, e.g. written by an
+ * animation kernel; no counterpart in the reference either): checked where
+ * they lie and copied over the scene's array on the device, with no host
+ * copy and no staging array.  Waits for work queued on the device; the
+ * caller's array may be reused once the call returns.  Rejects an array that
+ * is not device memory. */
+rtq_status rtq_scene_update_device(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle* tris_dev, uint32_t n_tris);
+
 /* ---- OBJ ingest (host only; no context or device needed) ------------------ */
 
 /* Mesh::load (mesh.cpp:186-263): the triangles of a Wavefront OBJ file in

@@ -413,10 +413,11 @@ static rtq_status check_models(rtq_ctx* ctx, const char* what, const rtq_mesh_mo
 }
 
 // The triangles are checked on the device after their copy (dev, n of them;
-// host = the caller's array, for the message): one pass over HBM instead of
-// a second pass over host memory.  Waits for the stream; the message names
-// the lowest failing triangle and, as a host loop over the triangles in order
-// would, its model id first.
+// host = the caller's array for the message, or null when the triangles came
+// from device memory): one pass over HBM instead of a second pass over host
+// memory.  Waits for the stream; the message names the lowest failing
+// triangle and, as a host loop over the triangles in order would, its model
+// id first.
 static rtq_status check_tris_on_device(rtq_ctx* ctx, const char* what, const rtq_triangle* host,
                                        const rtq_triangle* dev, uint32_t n, uint32_t n_models, uint32_t* words,
                                        hipStream_t st) {
@@ -430,8 +431,16 @@ static rtq_status check_tris_on_device(rtq_ctx* ctx, const char* what, const rtq
     RTQ_HIP(ctx, rd.wait());
     const uint32_t i = std::min(bad[0], bad[1]);
     if (i == kNone) return RTQ_OK;
-    if (bad[0] == i)
-        return fail(ctx, RTQ_EINVAL, "%s: triangle %u model id %u out of range", what, i, host[i].model_id);
+    if (bad[0] == i) {
+        uint32_t model_id = 0;
+        if (host) {
+            model_id = host[i].model_id;
+        } else {
+            RTQ_HIP(ctx, rd.add(&model_id, &dev[i].model_id, sizeof model_id));
+            RTQ_HIP(ctx, rd.wait());
+        }
+        return fail(ctx, RTQ_EINVAL, "%s: triangle %u model id %u out of range", what, i, model_id);
+    }
     return fail(ctx, RTQ_EINVAL, "%s: triangle %u has a non-finite or huge vertex", what, i);
 }
 
@@ -510,6 +519,40 @@ rtq_status rtq_scene_update(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle*
     if (models)
         RTQ_HIP(ctx, hipMemcpyAsync(s->models, models, sizeof(DevModel) * s->n_models, hipMemcpyHostToDevice,
                                     ctx->stream));
+    hipLaunchKernelGGL(k_world, dim3(blocks_for(s->n_tris)), dim3(kThreads), 0, ctx->stream, s->tris_ref, s->models,
+                       s->n_tris, s->tri_world);
+    RTQ_HIP(ctx, hipGetLastError());
+    RTQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (s->built) {
+        s->built = false;
+        s->stale = true;
+    }
+    return RTQ_OK;
+}
+
+rtq_status rtq_scene_update_device(rtq_ctx* ctx, uint32_t scene_id, const rtq_triangle* tris_dev, uint32_t n_tris) {
+    if (!ctx) return RTQ_EINVAL;
+    Scene* s = find_scene(ctx, scene_id);
+    if (!s) return fail(ctx, RTQ_EINVAL, "scene_update_device: unknown scene %u", scene_id);
+    if (!tris_dev) return fail(ctx, RTQ_EINVAL, "scene_update_device: null pointer");
+    if (n_tris != s->n_tris)
+        return fail(ctx, RTQ_EINVAL, "scene_update_device: %u triangles, the scene has %u", n_tris, s->n_tris);
+    RTQ_HIP(ctx, hipSetDevice(ctx->device));
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, tris_dev) != hipSuccess || attr.type != hipMemoryTypeDevice ||
+        attr.device != ctx->device) {
+        (void)hipGetLastError();
+        return fail(ctx, RTQ_EINVAL, "scene_update_device: the triangles are not in device memory of device %d",
+                    ctx->device);
+    }
+    // work already queued on the scene may still read the arrays
+    RTQ_HIP(ctx, hipDeviceSynchronize());
+    // checked where they lie, so a rejected update leaves the scene as it was
+    const rtq_status rs = check_tris_on_device(ctx, "scene_update_device", nullptr, tris_dev, s->n_tris, s->n_models,
+                                               s->check_words, ctx->stream);
+    if (rs != RTQ_OK) return rs;
+    RTQ_HIP(ctx, hipMemcpyAsync(s->tris_ref, tris_dev, sizeof(rtq_triangle) * s->n_tris, hipMemcpyDeviceToDevice,
+                                ctx->stream));
     hipLaunchKernelGGL(k_world, dim3(blocks_for(s->n_tris)), dim3(kThreads), 0, ctx->stream, s->tris_ref, s->models,
                        s->n_tris, s->tri_world);
     RTQ_HIP(ctx, hipGetLastError());

@@ -57,7 +57,7 @@ EXPORTED = [
     "rtq_order_model_create", "rtq_order_model_destroy", "rtq_order_model_log", "rtq_order_model_clear_samples",
     "rtq_order_model_train", "rtq_order_model_apply", "rtq_order_model_save", "rtq_order_model_load",
     "rtq_order_model_info", "rtq_shade", "rtq_bvh_overlay", "rtq_obj_load", "rtq_obj_parse",
-    "rtq_scene_update", "rtq_bvh_refit", "rtq_trace_noalloc", "rtq_bvh_export_wide",
+    "rtq_scene_update", "rtq_bvh_refit", "rtq_trace_noalloc", "rtq_bvh_export_wide", "rtq_scene_update_device",
 ]
 MAX_BATCH_ITEMS = 64
 
@@ -133,6 +133,7 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
         "rtq_scene_upload": (st, [vp, vp, u32, vp, u32, ctypes.POINTER(u32)]),
         "rtq_scene_release": (st, [vp, u32]),
         "rtq_scene_update": (st, [vp, u32, vp, u32, vp, u32]),
+        "rtq_scene_update_device": (st, [vp, u32, vp, u32]),
         "rtq_bvh_refit": (st, [vp, u32]),
         "rtq_build_params_default": (None, [ctypes.POINTER(BuildParams)]),
         "rtq_bvh_build": (st, [vp, u32, ctypes.POINTER(BuildParams)]),
@@ -415,6 +416,10 @@ class Scene:
         m = None if models is None else np.ascontiguousarray(models, dtype=MODEL_DTYPE)
         self.ctx.check(self.ctx.lib.rtq_scene_update(self.ctx.h, self.id, _ptr(t), 0 if t is None else len(t),
                                                      _ptr(m), 0 if m is None else len(m)))
+
+    def update_device(self, tris_ptr: int, n: int) -> None:
+        """New vertices already in device memory (n 64-B triangle records at tris_ptr); stale until refit or build."""
+        self.ctx.check(self.ctx.lib.rtq_scene_update_device(self.ctx.h, self.id, tris_ptr, n))
 
     def refit(self) -> None:
         """New boxes for the kept tree (rtq_bvh_refit)."""

@@ -175,6 +175,31 @@ int main() {
         (void)hipFree(d_m);
         (void)hipFree(d_px);
         bvh.update(std::vector<TriangleGPU>{}, models);   // back to y = 0 for the checks below
+
+        // the grid's vertices raised to y = 0.5 in device memory (as an
+        // animation kernel would write them), then back from the host
+        std::vector<TriangleGPU> raised = tris;
+        for (TriangleGPU& t : raised) {
+            t.p0.y += 0.5f;
+            t.p1.y += 0.5f;
+            t.p2.y += 0.5f;
+        }
+        TriangleGPU* d_tris = nullptr;
+        CHECK(hipMalloc(&d_tris, raised.size() * sizeof(TriangleGPU)) == hipSuccess);
+        CHECK(hipMemcpy(d_tris, raised.data(), raised.size() * sizeof(TriangleGPU), hipMemcpyHostToDevice) ==
+              hipSuccess);
+        bvh.update_device(reinterpret_cast<const rtq_triangle*>(d_tris), n);
+        for (const rtq_hit& h : bvh.trace(rays)) CHECK(h.id >= 0 && h.t == 0.5f);
+        threw = false;
+        try {
+            bvh.update_device(reinterpret_cast<const rtq_triangle*>(raised.data()), n);   // host memory
+        } catch (const rtq::Error& e) {
+            threw = (e.status == RTQ_EINVAL);
+        }
+        CHECK(threw);
+        (void)hipFree(d_tris);
+        bvh.update(tris, std::vector<MeshModelGPU>{});
+        for (const rtq_hit& h : bvh.trace(rays)) CHECK(h.id >= 0 && h.t == 1.0f);
     }
 
     // ---- frame batch + autotuner + learned order on device buffers --------

@@ -245,6 +245,53 @@ def test_upload_and_update_input_checks(rtq_ctx):
 
 
 @pytest.mark.gpu
+def test_update_from_device_memory(rtq_ctx, torch_mod):
+    """rtq_scene_update_device: triangles already in device memory give the
+    same scene as the host update (hits after a refit equal brute force and
+    the host path's), with the same checks and messages; host memory and a
+    wrong count are refused before any copy."""
+    models = scenes.identity_models()
+    good = scenes.soup(3000, seed=41)
+    sc = rtq_ctx.upload(good, models)
+    try:
+        sc.build(16, 4, 0, node_width=8)
+        rays = rays_for(4096, seed=43)
+        before = sc.trace_host(rays)
+        new = moved(good, seed=44)
+        d_new = devmem.to_device(new)
+        torch_mod.cuda.synchronize()
+        sc.update_device(d_new.data_ptr(), len(new))
+        sc.refit()
+        got_dev = sc.trace_host(rays)
+        ref = oracle.trace_brute(oracle.world_vertices(new, models), rays)
+        assert hits_equal(got_dev, ref), first_diff(got_dev, ref)
+        sc.update(good)                                  # back, then the same edit from the host
+        sc.refit()
+        assert np.array_equal(sc.trace_host(rays).view(np.uint8), before.view(np.uint8))
+        sc.update(new)
+        sc.refit()
+        assert np.array_equal(sc.trace_host(rays).view(np.uint8), got_dev.view(np.uint8))
+        # rejected: bad records (scene unchanged), host memory, wrong count
+        bad = new.copy()
+        bad["model_id"][2500] = 3
+        bad["p0"][2600, 1] = np.nan
+        d_bad = devmem.to_device(bad)
+        torch_mod.cuda.synchronize()
+        with pytest.raises(RtqError) as e:
+            sc.update_device(d_bad.data_ptr(), len(bad))
+        assert e.value.status == RTQ_EINVAL and "triangle 2500 model id 3 out of range" in str(e.value)
+        with pytest.raises(RtqError) as e:
+            sc.update_device(bad.ctypes.data, len(bad))
+        assert e.value.status == RTQ_EINVAL and "not in device memory" in str(e.value)
+        with pytest.raises(RtqError) as e:
+            sc.update_device(d_new.data_ptr(), len(new) - 1)
+        assert e.value.status == RTQ_EINVAL
+        assert np.array_equal(sc.trace_host(rays).view(np.uint8), got_dev.view(np.uint8))
+    finally:
+        sc.release()
+
+
+@pytest.mark.gpu
 def test_batch_refuses_stale_or_rebuilt_scene(rtq_ctx, torch_mod):
     """A batch graph holds its scenes' node buffers: it replays after a refit
     (buffers kept), and refuses after an edit without refit or a rebuild."""

@@ -415,6 +415,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             const uint32_t ny0 = py ? qy.x : qy.z, ny1 = py ? qy.y : qy.w, fy0 = py ? qy.z : qy.x, fy1 = py ? qy.w : qy.y;
             const uint32_t nz0 = pz ? qz.x : qz.z, nz1 = pz ? qz.y : qz.w, fz0 = pz ? qz.z : qz.x, fz1 = pz ? qz.w : qz.y;
             uint32_t key[8];
+            uint32_t hm = 0;   // slots hit, the learned first child left out
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int w = j >> 2, sh = 8 * (j & 3);
@@ -432,19 +433,19 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 // order key: entry distance (non-negative floats order as uints), slot in the low bits
                 key[j] = (hit && !learned_first) ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j)
                                                  : 0xFFFFFFFFu;
+                hm |= (hit && !learned_first ? 1u : 0u) << j;
             }
             if (MODE == 1 || (MODE == 2 && lane_any)) {
-                // any hit: nearest (or the learned) child first, the others in slot order
-                uint32_t m = min(min(min(key[0], key[1]), min(key[2], key[3])),
-                                 min(min(key[4], key[5]), min(key[6], key[7])));
-                if (!lf_hit && m == 0xFFFFFFFFu) {
+                // any hit: the learned child first, else the lowest hit slot, the others in slot order
+                // (entry-distance order cost more VALU than it saved visits, occluded rays included)
+                if (!lf_hit && hm == 0u) {
                     cur = pop();
                     return;
                 }
-                const uint32_t first = lf_hit ? lf : (m & 7u);
+                const uint32_t first = lf_hit ? lf : (uint32_t)__builtin_ctz(hm);
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
-                    if (key[j] != 0xFFFFFFFFu && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                    if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
                 cur = slot_ref(c03, c47, first);
             } else {
                 // the learned child (not in the keys) goes first, the others nearest first

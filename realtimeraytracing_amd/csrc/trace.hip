@@ -397,7 +397,6 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             const float bx = __builtin_fmaf(h.x, ix, -oix);
             const float by = __builtin_fmaf(h.y, iy, -oiy);
             const float bz = __builtin_fmaf(h.z, iz, -oiz);
-            const uint32_t valid = meta >> 24;
             // learned first child for the ray's octant and mode (table entry
             // 8 | slot; lf = 8: none); dropped from the key list when hit
             uint32_t lf = 8u;
@@ -427,12 +426,15 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 const float tfz = __builtin_fmaf((float)(((w ? fz1 : fz0) >> sh) & 0xFFu), scz, bz);
                 const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
                 const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tbest));
-                const bool hit = (tn <= __builtin_fmaf(fabsf(tf), kSlabRel, tf)) && ((valid >> j) & 1u);
+                // empty slots hold inverted boxes and a harmless leaf ref (one
+                // real triangle), so the valid bits need no test here
+                const bool hit = tn <= __builtin_fmaf(fabsf(tf), kSlabRel, tf);
                 const bool learned_first = LAYOUT == kWideOrdered && hit && lf == (uint32_t)j;
                 if (LAYOUT == kWideOrdered) lf_hit = lf_hit || learned_first;
                 // order key: entry distance (non-negative floats order as uints), slot in the low bits
-                key[j] = (hit && !learned_first) ? ((__float_as_uint(fmaxf(tn, 0.0f)) & ~7u) | (uint32_t)j)
-                                                 : 0xFFFFFFFFu;
+                key[j] = (hit && !learned_first)
+                             ? (((uint32_t)max((int32_t)__float_as_uint(tn), 0) & ~7u) | (uint32_t)j)
+                             : 0xFFFFFFFFu;
                 hm |= (hit && !learned_first ? 1u : 0u) << j;
             }
             if (MODE == 1 || (MODE == 2 && lane_any)) {

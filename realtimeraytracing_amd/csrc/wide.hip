@@ -251,6 +251,12 @@ __device__ void write_wide_boxes(const uint32_t* ch, uint32_t n, const rtq_aabb*
             qhi[q][i >> 2] |= (uint32_t)h << (8 * (i & 3));
         }
     }
+    // empty slots: an inverted box (lower planes 255, upper 0) that a ray
+    // misses on every axis unless its origin lies more than 2^19 node extents
+    // away; their child ref is then harmless (one triangle test, see the
+    // level kernel), so traversal needs no test of the valid bits
+    for (uint32_t i = n; i < kWidth; ++i)
+        for (int q = 0; q < 3; ++q) qlo[q][i >> 2] |= 0xFFu << (8 * (i & 3));
     const uint32_t meta = (uint32_t)(ex[0] + 127) | ((uint32_t)(ex[1] + 127) << 8) |
                           ((uint32_t)(ex[2] + 127) << 16) | (valid << 24);
     o[0] = make_float4(org[0], org[1], org[2], __uint_as_float(meta));
@@ -419,7 +425,10 @@ __global__ __launch_bounds__(kThreads) void k_wide_level(LevelArgs a) {
     uint32_t ref[kWidth];
     uint32_t rank = 0;
     for (uint32_t i = 0; i < kWidth; ++i) {
-        ref[i] = 0u;
+        // an empty slot's box is inverted (write_wide_boxes); should a far ray
+        // still enter it, its ref is the leaf of traversal triangle 0 alone:
+        // a real triangle, so whatever that test reports is a genuine hit
+        ref[i] = leaf_ref(0u, 1u);
         if (i >= n) continue;
         const uint32_t c = ch[i];
         if ((expand >> i) & 1u) {

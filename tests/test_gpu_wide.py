@@ -221,6 +221,18 @@ def test_wide_collapse_follows_cost_table(rtq_ctx, n, leaf):
         refs = [int(raw[k, 16 + j]) for k in range(len(raw)) for j in range(8)
                 if (int(raw[k, 3]) >> 24 >> j) & 1 and not int(raw[k, 16 + j]) & 0x80000000]
         assert refs == list(range(1, len(raw)))
+        # empty slots: inverted boxes (lower planes 255, upper 0 on every axis)
+        # and the one-triangle leaf of traversal triangle 0 as their ref
+        # [node, axis, lower/upper, word, byte]
+        planes = np.ascontiguousarray(raw[:, 4:16]).view(np.uint8).reshape(len(raw), 3, 2, 2, 4)
+        for k in range(len(raw)):
+            valid = int(raw[k, 3]) >> 24
+            for j in range(8):
+                if (valid >> j) & 1:
+                    continue
+                assert int(raw[k, 16 + j]) == 0x80000000, (k, j)
+                lo, hi = planes[k, :, 0, j >> 2, j & 3], planes[k, :, 1, j >> 2, j & 3]
+                assert (lo == 255).all() and (hi == 0).all(), (k, j, lo, hi)
     finally:
         sc.release()
 

@@ -149,9 +149,9 @@ rtq_status rtq_bvh_build(rtq_ctx* ctx, uint32_t scene_id, const rtq_build_params
 /* Refit: the built tree's topology kept (PLOC merges, leaf order, binary
  * traversal node indices), every box recomputed bottom-up for the scene's
  * current vertices (leaf = triangle box, bvh.cpp:383-399; parent = merge of its
- * children, bvh.cpp:401-413), then the reference node array, the traversal
- * nodes and triangles rewritten in their buffers, so batch graphs captured
- * before stay valid.  The 8-wide nodes keep their children (node count, depth,
+ * children, bvh.cpp:401-413), then the traversal nodes and triangles
+ * rewritten in their buffers, so batch graphs captured before stay valid (the
+ * reference node array is rewritten on the next export).  The 8-wide nodes keep their children (node count, depth,
  * child refs) and get boxes quantised from the new cluster boxes.  Learned
  * order words and tables are cleared.  Hit records stay exact (they never
  * depend on the tree); traversal cost grows as the vertices move away from

@@ -101,6 +101,7 @@ void free_build(Scene* s) {
     release(s->preorder);
     release(s->first_leaf);
     release(s->node_depth);
+    release(s->trav_index);
     release(s->top);
     release(s->tris);
 }

@@ -1361,15 +1361,17 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     if (!s->preorder) RTQ_HIP(ctx, dev_alloc(&s->preorder, nc));
     if (!s->first_leaf) RTQ_HIP(ctx, dev_alloc(&s->first_leaf, nc));
     if (!s->node_depth) RTQ_HIP(ctx, dev_alloc(&s->node_depth, nc));
+    // so is every cluster's traversal node index: a refit needs no new marks
+    if (!s->trav_index) RTQ_HIP(ctx, dev_alloc(&s->trav_index, nc));
     uint32_t* const pos = s->preorder;
     uint32_t* const fl = s->first_leaf;
     uint32_t* const dep = s->node_depth;
+    bidx = s->trav_index;
     s->ref_ready = false;
     RTQ_HIP(ctx, tmp.get(&maxd, 2));
     uint32_t* bdep = nullptr;
     RTQ_HIP(ctx, tmp.get(&bdep, blocks_for(nc)));
     RTQ_HIP(ctx, tmp.get(&flag, nc));
-    RTQ_HIP(ctx, tmp.get(&bidx, nc));
     RTQ_HIP(ctx, tmp.get(&ntrav, 1));
     const uint32_t root = nc - 1;
     // The 8-wide collapse's cost table reads the clusters only: it runs on the
@@ -1418,23 +1420,28 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, s->ctri, s->cleft, fl,
                        dep, s->tri_world, s->tris, bdep);
     RTQ_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, bdep, blocks_for(nc), maxd);
-    RTQ_HIP(ctx, hipGetLastError());
-    if (n > 1) {
-        hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root, s->params.max_leaf_size,
-                           s->cleft, s->ccount, pos, flag);
+    // the depth and the traversal node indices depend on the topology only:
+    // a refit keeps them
+    if (!refit) {
+        hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, st, bdep, blocks_for(nc), maxd);
         RTQ_HIP(ctx, hipGetLastError());
-        rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
-        if (rs != RTQ_OK) return rs;
+        if (n > 1) {
+            hipLaunchKernelGGL(k_mark_trav, dim3(blocks_for(nc)), dim3(kThreads), 0, st, nc, root,
+                               s->params.max_leaf_size, s->cleft, s->ccount, pos, flag);
+            RTQ_HIP(ctx, hipGetLastError());
+            rs = scan_u32(ctx, flag, bidx, nc, ntrav, st);
+            if (rs != RTQ_OK) return rs;
+        }
     }
 
-    // one host read: scene bounds (box pad), tree depth, traversal node count
+    // one host read: scene bounds (box pad), and after a build the tree depth
+    // and the traversal node count
     rtq_aabb h_bounds[2];
-    uint32_t h_depth = 0, h_ntrav = 1;
+    uint32_t h_depth = s->depth, h_ntrav = refit ? s->n_trav : 1u;
     HostReads rd(ctx, st);
     RTQ_HIP(ctx, rd.add(h_bounds, d_bounds, sizeof h_bounds));
-    RTQ_HIP(ctx, rd.add(&h_depth, maxd, sizeof h_depth));
-    if (n > 1) RTQ_HIP(ctx, rd.add(&h_ntrav, ntrav, sizeof h_ntrav));
+    if (!refit) RTQ_HIP(ctx, rd.add(&h_depth, maxd, sizeof h_depth));
+    if (!refit && n > 1) RTQ_HIP(ctx, rd.add(&h_ntrav, ntrav, sizeof h_ntrav));
     if (flag_dev) RTQ_HIP(ctx, rd.add(flag_host, flag_dev, sizeof(uint32_t)));
     rs = queue_cost(dp_steps);
     if (rs != RTQ_OK) return rs;

@@ -58,6 +58,7 @@ struct Scene {
     uint32_t* preorder = nullptr;       // pre-order position of every cluster (2N - 1)
     uint32_t* first_leaf = nullptr;     // rank of the first leaf below every cluster (2N - 1)
     uint32_t* node_depth = nullptr;     // depth of every cluster (2N - 1)
+    uint32_t* trav_index = nullptr;     // binary traversal node of every cluster (2N - 1), kNone if none
     float4* nodes = nullptr;            // traversal nodes, 4 float4 (64 B) each
     uint4* node_aux = nullptr;          // per traversal node: child leaf ranges [A,B) [B,C), depth
     bool ordered = false;               // learned child-order words stored in the nodes

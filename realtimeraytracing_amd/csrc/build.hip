@@ -1439,7 +1439,7 @@ rtq_status write_layouts(rtq_ctx* ctx, Scene* s, const rtq_aabb* d_bounds, hipSt
     rtq_aabb h_bounds[2];
     uint32_t h_depth = s->depth, h_ntrav = refit ? s->n_trav : 1u;
     HostReads rd(ctx, st);
-    RTQ_HIP(ctx, rd.add(h_bounds, d_bounds, sizeof h_bounds));
+    RTQ_HIP(ctx, rd.add(h_bounds, d_bounds, refit ? sizeof(rtq_aabb) : sizeof h_bounds));   // a refit: the root box
     if (!refit) RTQ_HIP(ctx, rd.add(&h_depth, maxd, sizeof h_depth));
     if (!refit && n > 1) RTQ_HIP(ctx, rd.add(&h_ntrav, ntrav, sizeof h_ntrav));
     if (flag_dev) RTQ_HIP(ctx, rd.add(flag_host, flag_dev, sizeof(uint32_t)));
@@ -1638,9 +1638,6 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
     const uint32_t n = s->n_tris;
     const rtq_status ra = arena_reserve(ctx);
     if (ra != RTQ_OK) return ra;
-    Temps tmp(ctx);
-    rtq_aabb* d_bounds = nullptr;
-    RTQ_HIP(ctx, tmp.get(&d_bounds, kBoundsBoxes));
     hipLaunchKernelGGL(k_refit_leaves, dim3(blocks_for(n)), dim3(kThreads), 0, st, s->tris_ref, s->models, s->ctri,
                        n, s->cbox);
     RTQ_HIP(ctx, hipGetLastError());
@@ -1656,9 +1653,10 @@ rtq_status refit_scene(rtq_ctx* ctx, Scene* s) {
                            s->passes[s->lds_from].first, s->n_clusters, s->cleft, s->cright, s->cbox);
         RTQ_HIP(ctx, hipGetLastError());
     }
-    rtq_status rs = compute_bounds(ctx, s, s->params.flags & RTQ_BUILD_REF_PADDED_BOUNDS, d_bounds, st);
-    if (rs != RTQ_OK) return rs;
-    return write_layouts(ctx, s, d_bounds, st, true);
+    // the box pad needs the largest coordinate magnitude of the scene bounds:
+    // the root cluster's new box holds the same minima and maxima (merges are
+    // exact), and the reference quirk of adding the origin cannot raise it
+    return write_layouts(ctx, s, s->cbox + (s->n_clusters - 1), st, true);
 }
 
 rtq_status morton_codes(rtq_ctx* ctx, Scene* s, uint32_t flags, uint32_t* host_codes) {

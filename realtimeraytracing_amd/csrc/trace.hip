@@ -125,6 +125,19 @@ struct TraceArgs {
     uint32_t n_top;      // its node count (0: traversal starts at global node 0)
 };
 
+// Rays are read once and hit records written once per launch: non-temporal
+// (streaming) accesses keep them from displacing nodes and triangles in the
+// caches.
+typedef float stream4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float4* p) {
+    const stream4 v = __builtin_nontemporal_load(reinterpret_cast<const stream4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_stream(rtq_hit* p, const rtq_hit& h) {
+    const stream4 v = {__int_as_float(h.id), h.t, h.u, h.v};
+    __builtin_nontemporal_store(v, reinterpret_cast<stream4*>(p));
+}
+
 __device__ __forceinline__ float sel3(float4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 __device__ __forceinline__ float sel3f(float x, float y, float z, int k) {
     return k == 0 ? x : (k == 1 ? y : z);
@@ -371,9 +384,9 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
         }
         if constexpr (MULTI != 0) {
             const BatchItem& it = s_items[lane_item];
-            it.hits[(uint32_t)rid - it.begin] = h;
+            store_stream(it.hits + ((uint32_t)rid - it.begin), h);
         } else {
-            hits_p[local] = h;
+            store_stream(hits_p + local, h);
         }
         if (STATS) {
             if (a.stat_nodes) a.stat_nodes[rid] = n_nodes;
@@ -593,7 +606,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                         order_mask = it.learned ? 0xFFFFFFFFu : 0u;
                     }
                     const float4* rp = reinterpret_cast<const float4*>(rays_p + local);
-                    const float4 r0 = rp[0], r1 = rp[1];
+                    const float4 r0 = load_stream(rp), r1 = load_stream(rp + 1);
                     ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
                     dx = r1.x; dy = r1.y; dz = r1.z; tbest = r1.w;
                     if constexpr (learned_layout(LAYOUT)) {

@@ -20,12 +20,22 @@ class OrderModel:
         ctx.order_init(seed, self.params.data_ptr())
 
     def log(self, scene: Scene, rays, n: int, any_hit: bool, stride: int = 1, capacity: int = 1 << 20):
-        """Samples from every `stride`-th ray of a device ray buffer: (tensor [k, ORDER_SAMPLE], k)."""
+        """Samples from every `stride`-th ray of a device ray buffer: (tensor [k, ORDER_SAMPLE], k).
+
+        The kernel appends samples in the order its atomics hand out slots;
+        they are returned sorted by a key computed from their contents, so the
+        same rays give the same sample order and training is reproducible."""
         import torch
         buf = torch.empty((capacity, ORDER_SAMPLE), dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
         k = scene.order_log(rays.data_ptr(), n, buf.data_ptr(), capacity, RTQ_TRACE_ANY if any_hit else 0, stride)
-        return buf[:k].contiguous(), k
+        out = buf[:k]
+        if k > 1:
+            g = torch.Generator(device="cpu").manual_seed(0x5EED)
+            w = torch.rand(ORDER_SAMPLE, generator=g, dtype=torch.float64).to("cuda") + 0.5
+            key = (out.double() * w).sum(dim=1)
+            out = out[torch.argsort(key, stable=True)]
+        return out.contiguous(), k
 
     def train(self, samples, epochs: int = 4, batch: int = 1024, lr: float = 2e-3, seed: int = 7):
         """Adam epochs over a [k, ORDER_SAMPLE] device tensor; returns the per-epoch mean losses."""

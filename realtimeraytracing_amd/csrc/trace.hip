@@ -451,16 +451,24 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 hm |= (hit && !learned_first ? 1u : 0u) << j;
             }
             if (MODE == 1 || (MODE == 2 && lane_any)) {
-                // any hit: the learned child first, else the lowest hit slot, the others in slot order
-                // (entry-distance order cost more VALU than it saved visits, occluded rays included)
+                // any hit: the learned child first, else the lowest hit slot; the others in slot order,
+                // popped outward from the first child's side (slots follow the binary subtree's
+                // left-to-right frontier, so neighbouring slots are neighbouring subtrees); an
+                // entry-distance order cost more VALU than it saved visits, occluded rays included
                 if (!lf_hit && hm == 0u) {
                     cur = pop();
                     return;
                 }
                 const uint32_t first = lf_hit ? lf : (uint32_t)__builtin_ctz(hm);
+                if (!lf_hit || lf < 4u) {
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                    for (int j = 7; j >= 0; --j)   // the highest pushed first: pops go up
+                        if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)    // pops go down from a learned child in the upper half
+                        if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                }
                 cur = slot_ref(c03, c47, first);
             } else {
                 // the learned child (not in the keys) goes first, the others nearest first

@@ -459,8 +459,22 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                     cur = pop();
                     return;
                 }
-                const uint32_t first = lf_hit ? lf : (uint32_t)__builtin_ctz(hm);
-                if (!lf_hit || lf < 4u) {
+                if (!lf_hit) {
+                    // the highest pushed first, so pops go up; the lowest hit slot is
+                    // kept back as the next node (its ref without a select tree)
+                    const uint32_t refs[8] = {c03.x, c03.y, c03.z, c03.w, c47.x, c47.y, c47.z, c47.w};
+                    uint32_t next = kEmpty;
+#pragma unroll
+                    for (int j = 7; j >= 0; --j)
+                        if ((hm >> j) & 1u) {
+                            if (next != kEmpty) push(next);
+                            next = refs[j];
+                        }
+                    cur = next;
+                    return;
+                }
+                const uint32_t first = lf;
+                if (lf < 4u) {
 #pragma unroll
                     for (int j = 7; j >= 0; --j)   // the highest pushed first: pops go up
                         if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));

@@ -427,7 +427,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
             const uint32_t ny0 = py ? qy.x : qy.z, ny1 = py ? qy.y : qy.w, fy0 = py ? qy.z : qy.x, fy1 = py ? qy.w : qy.y;
             const uint32_t nz0 = pz ? qz.x : qz.z, nz1 = pz ? qz.y : qz.w, fz0 = pz ? qz.z : qz.x, fz1 = pz ? qz.w : qz.y;
             uint32_t key[8];
-            uint32_t hm = 0;   // slots hit, the learned first child left out
+            bool hv[8];   // slot hit, the learned first child left out (kept as lane masks: no VALU to test)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int w = j >> 2, sh = 8 * (j & 3);
@@ -448,14 +448,14 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 key[j] = (hit && !learned_first)
                              ? (((uint32_t)max((int32_t)__float_as_uint(tn), 0) & ~7u) | (uint32_t)j)
                              : 0xFFFFFFFFu;
-                hm |= (hit && !learned_first ? 1u : 0u) << j;
+                hv[j] = hit && !learned_first;
             }
             if (MODE == 1 || (MODE == 2 && lane_any)) {
                 // any hit: the learned child first, else the lowest hit slot; the others in slot order,
                 // popped outward from the first child's side (slots follow the binary subtree's
                 // left-to-right frontier, so neighbouring slots are neighbouring subtrees); an
                 // entry-distance order cost more VALU than it saved visits, occluded rays included
-                if (!lf_hit && hm == 0u) {
+                if (!lf_hit && !(hv[0] || hv[1] || hv[2] || hv[3] || hv[4] || hv[5] || hv[6] || hv[7])) {
                     cur = pop();
                     return;
                 }
@@ -466,7 +466,7 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                     uint32_t next = kEmpty;
 #pragma unroll
                     for (int j = 7; j >= 0; --j)
-                        if ((hm >> j) & 1u) {
+                        if (hv[j]) {
                             if (next != kEmpty) push(next);
                             next = refs[j];
                         }
@@ -477,11 +477,11 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                 if (lf < 4u) {
 #pragma unroll
                     for (int j = 7; j >= 0; --j)   // the highest pushed first: pops go up
-                        if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                        if (hv[j] && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
                 } else {
 #pragma unroll
                     for (int j = 0; j < 8; ++j)    // pops go down from a learned child in the upper half
-                        if (((hm >> j) & 1u) && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
+                        if (hv[j] && (uint32_t)j != first) push(slot_ref(c03, c47, (uint32_t)j));
                 }
                 cur = slot_ref(c03, c47, first);
             } else {

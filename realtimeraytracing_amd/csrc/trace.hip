@@ -464,11 +464,13 @@ __device__ __forceinline__ void trace_waves(const TraceArgs& a, const ItemTable<
                     // kept back as the next node (its ref without a select tree)
                     const uint32_t refs[8] = {c03.x, c03.y, c03.z, c03.w, c47.x, c47.y, c47.z, c47.w};
                     uint32_t next = kEmpty;
+                    bool have = false;   // a lane mask: no compare of `next`
 #pragma unroll
                     for (int j = 7; j >= 0; --j)
                         if (hv[j]) {
-                            if (next != kEmpty) push(next);
+                            if (have) push(next);
                             next = refs[j];
+                            have = true;
                         }
                     cur = next;
                     return;
